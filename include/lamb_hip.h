@@ -1,0 +1,162 @@
+/*
+ * lamb_hip.h -- C-ABI of the MI355X (gfx950) sector block-sparse multiply.
+ *
+ * Drop-in boundary for the reference's Fortran 2003 multiplication path.
+ * Every entry point is extern "C", takes plain pointers and fixed-width
+ * integers, and returns an int32 status: 0 = ok, otherwise the error level
+ * of the reference's error module (src/lamb_error.F90:116-137):
+ *   1 note, 2 warning, 3 failure, 4 memory, 5 fatal.
+ * The error *type* and a message are available from lamb_hip_last_error;
+ * types follow src/lamb_error.F90:141-159 (caller 1, wrong_args 100,
+ * precondition 200, internal -1, overflow -2, unimplemented -1000,
+ * external -2000).
+ *
+ * Integer widths follow the reference's kinds (src/lamb_kinds.F90:155-179):
+ * allatm_k int64 (global atom counts), locatm_k int16 (subset-local atom
+ * index), subset_k int32, blksz_k int8 (block dimension), blkn_k int64
+ * (block counts / CSR offsets), dataptr_k int64 (data extents),
+ * space_k float32 (coordinates).
+ *
+ * Ownership: the caller owns every host array it passes; the library copies
+ * what it needs into device memory before returning and keeps no host
+ * pointer (the reference's data_X%mine = .true. semantics,
+ * src/lamb_data_X_types.F90:77-83). Device memory belongs to the handle
+ * that allocated it and is released by the matching _destroy.
+ *
+ * Threading: one host thread per context. Calls are ordered on the
+ * context's HIP stream and return after the device work they start has
+ * finished (blocking), like the reference's synchronous lamb_multiply.
+ */
+#ifndef LAMB_HIP_H
+#define LAMB_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lamb_hip_ctx lamb_hip_ctx;
+typedef struct lamb_hip_dist lamb_hip_dist;
+typedef struct lamb_hip_mat lamb_hip_mat;
+
+/* error levels, src/lamb_error.F90:116-137 */
+#define LAMB_HIP_OK 0
+#define LAMB_HIP_NOTE 1
+#define LAMB_HIP_WARNING 2
+#define LAMB_HIP_FAILURE 3
+#define LAMB_HIP_MEMORY 4
+#define LAMB_HIP_FATAL 5
+
+/* error types, src/lamb_error.F90:141-159 */
+#define LAMB_HIP_CALLER_ERROR 1
+#define LAMB_HIP_WRONG_ARGS 100
+#define LAMB_HIP_PRECONDITION_FAILED 200
+#define LAMB_HIP_INTERNAL_ERROR (-1)
+#define LAMB_HIP_OVERFLOW_ERROR (-2)
+#define LAMB_HIP_UNIMPLEMENTED (-1000)
+#define LAMB_HIP_EXTERNAL_ERROR (-2000)
+
+/* multiply flags */
+#define LAMB_HIP_FP64 0          /* plain FP64 block products */
+#define LAMB_HIP_BF16_SPLIT 1    /* exponent-aligned BF16 slices, FP32 MFMA */
+
+/* Statistics of one multiply (filled when a non-NULL pointer is given). */
+typedef struct lamb_hip_stats {
+  int64_t n_triples_kept;      /* atom-level (i,k,j) products kept by the prefilter */
+  int64_t n_triples_filtered;  /* stored (i,k),(k,j) pairs dropped by the prefilter */
+  int64_t n_c_blocks_struct;   /* C blocks in the structural pattern */
+  int64_t n_c_blocks_kept;     /* C blocks stored after the post-filter */
+  int64_t n_c_elems_kept;      /* FP64 values in the stored C blocks */
+  int64_t n_tiles;             /* C tiles processed by the numeric kernel */
+  int64_t n_tile_entries;      /* (tile, k) strip pairs processed */
+  double flops;                /* sum over kept triples of 2*m_i*k_k*n_j */
+  double ms_symbolic;          /* device time of the symbolic phase */
+  double ms_numeric;           /* device time of the numeric kernel */
+  double ms_index;             /* device time of the C index build */
+  double ms_total;             /* wall time of the whole call */
+} lamb_hip_stats;
+
+/* --- context ------------------------------------------------------------ */
+int32_t lamb_hip_init(int32_t device, lamb_hip_ctx** ctx);
+int32_t lamb_hip_finalize(lamb_hip_ctx* ctx);
+/* Last error of this context: level, type and a NUL-terminated message. */
+int32_t lamb_hip_last_error(const lamb_hip_ctx* ctx, int32_t* level, int32_t* type,
+                            char* msg, int32_t msg_len);
+
+/* --- distribution: atoms, their subsets and positions --------------------
+ * Replaces the atom set + set partition the reference builds in
+ * lamb_set_partition_update (src/lamb_set_partition_methods.F90:69-144):
+ * subset_map(g) is the 1-based subset of global atom g, order_map(g) its
+ * 1-based local index inside that subset. coords (3*n_atoms, xyz per atom)
+ * and box (3 edge lengths) are optional (NULL): when given, the device pool
+ * orders atoms along a recursive coordinate bisection so that MFMA tiles
+ * cover compact groups of atoms. */
+int32_t lamb_hip_dist_create(lamb_hip_ctx* ctx, int64_t n_atoms, const int8_t* blk_size,
+                             int32_t n_subsets, const int32_t* subset_map,
+                             const int16_t* order_map, const float* coords,
+                             const float* box, lamb_hip_dist** dist);
+int32_t lamb_hip_dist_destroy(lamb_hip_dist* dist);
+
+/* --- matrices ------------------------------------------------------------
+ * A square block matrix over one distribution (rows and columns share the
+ * atom set, as lamb_multiply requires of A, B, C,
+ * src/lamb_multiplication.F90:136-146). */
+int32_t lamb_hip_matrix_create(lamb_hip_dist* dist, lamb_hip_mat** mat);
+int32_t lamb_hip_matrix_destroy(lamb_hip_mat* mat);
+/* Device-side copy (lamb_matrix_copy, src/lamb_matrix_X_types.F90:144-187). */
+int32_t lamb_hip_matrix_copy(const lamb_hip_mat* src, lamb_hip_mat* dst);
+
+/* Stage one sector (subset_row, subset_col) in the reference's layout
+ * (lamb_sector_X_type, src/lamb_sector_X_types.F90:79-87; index
+ * src/lamb_index_types.F90:86-105): row_p int64[n_rows+1] 0-based
+ * exclusive offsets, col_i int16[n_blks] 1-based subset-local column atom,
+ * extents int64[2*n_blks] (lo, hi) 1-based inclusive into data, data the
+ * sector's FP64 store of data_len values, each block column-major.
+ * n_rows is the size of subset_row. Replaces the block insertion of
+ * fill_local_sectors (src/lamb_matrix_operations.F90:294-375). */
+int32_t lamb_hip_sector_upload(lamb_hip_mat* mat, int32_t subset_row, int32_t subset_col,
+                               int64_t n_blks, const int64_t* row_p, const int16_t* col_i,
+                               const int64_t* extents, const double* data, int64_t data_len);
+/* Block-list form of internal_distribute (src/lamb_matrix_operations.F90:88-97)
+ * for one data store: n blocks, subset_idx int32[2*n] (row, col subset),
+ * local_idx int16[2*n] (1-based local row/col atom), extents int64[2*n]
+ * 1-based inclusive into data. Blocks may arrive in any order. */
+int32_t lamb_hip_distribute(lamb_hip_mat* mat, int64_t n, const int32_t* subset_idx,
+                            const int16_t* local_idx, const int64_t* extents,
+                            const double* data, int64_t data_len);
+/* Build the device pool from everything staged since the last commit
+ * (replaces lamb_index_convert_to_csr, src/lamb_index_methods.F90:224-246). */
+int32_t lamb_hip_matrix_commit(lamb_hip_mat* mat);
+
+/* Sizes of sector (subset_row, subset_col) as it would be downloaded. */
+int32_t lamb_hip_sector_query(lamb_hip_mat* mat, int32_t subset_row, int32_t subset_col,
+                              int64_t* n_blks, int64_t* data_len);
+/* Fill caller-allocated arrays (sizes from lamb_hip_sector_query) with the
+ * sector in the reference's layout; col_i ascending per row, blocks stored
+ * back to back in row order. */
+int32_t lamb_hip_sector_download(lamb_hip_mat* mat, int32_t subset_row, int32_t subset_col,
+                                 int64_t* row_p, int16_t* col_i, int64_t* extents,
+                                 double* data);
+/* Totals of the whole matrix. */
+int32_t lamb_hip_matrix_info(const lamb_hip_mat* mat, int64_t* n_blks, int64_t* n_elems);
+
+/* --- operations ----------------------------------------------------------
+ * C = alpha * A*B + beta * C (lamb_multiply, src/lamb_multiplication.F90:116-204,
+ * whose per-sector work sector_multiply :206-225 this implements).
+ * Triple (i,k,j) is kept iff nsq(A_ik)*nsq(B_kj) >= eps_filter^2; C_ij is
+ * stored iff nsq(C_ij) >= eps_filter^2, where nsq is the sum of squares of a
+ * block's elements in column-major order, one fused multiply-add per step.
+ * beta == 0 ignores C's old contents. C may not alias A or B. */
+int32_t lamb_hip_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A, const lamb_hip_mat* B,
+                          lamb_hip_mat* C, double alpha, double beta, double eps_filter,
+                          int32_t flags, lamb_hip_stats* stats);
+
+/* tr(A): sum of the diagonal elements of the diagonal blocks (FP64). */
+int32_t lamb_hip_trace(const lamb_hip_mat* A, double* trace);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LAMB_HIP_H */

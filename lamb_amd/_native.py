@@ -1,0 +1,94 @@
+"""ctypes binding of liblamb_hip.so (declared in include/lamb_hip.h).
+
+The library is the only compute path: there is no CPU fallback. Loading
+fails with a RuntimeError when the shared object has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+
+LIB_PATH = pathlib.Path(__file__).resolve().parent / "liblamb_hip.so"
+
+i8p = C.POINTER(C.c_int8)
+i16p = C.POINTER(C.c_int16)
+i32p = C.POINTER(C.c_int32)
+i64p = C.POINTER(C.c_int64)
+f32p = C.POINTER(C.c_float)
+f64p = C.POINTER(C.c_double)
+vpp = C.POINTER(C.c_void_p)
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("n_triples_kept", C.c_int64),
+        ("n_triples_filtered", C.c_int64),
+        ("n_c_blocks_struct", C.c_int64),
+        ("n_c_blocks_kept", C.c_int64),
+        ("n_c_elems_kept", C.c_int64),
+        ("n_tiles", C.c_int64),
+        ("n_tile_entries", C.c_int64),
+        ("flops", C.c_double),
+        ("ms_symbolic", C.c_double),
+        ("ms_numeric", C.c_double),
+        ("ms_index", C.c_double),
+        ("ms_total", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "lamb_hip_init": (C.c_int32, [C.c_int32, vpp]),
+    "lamb_hip_finalize": (C.c_int32, [C.c_void_p]),
+    "lamb_hip_last_error": (C.c_int32, [C.c_void_p, i32p, i32p, C.c_char_p, C.c_int32]),
+    "lamb_hip_dist_create": (C.c_int32, [C.c_void_p, C.c_int64, i8p, C.c_int32, i32p, i16p,
+                                         f32p, f32p, vpp]),
+    "lamb_hip_dist_destroy": (C.c_int32, [C.c_void_p]),
+    "lamb_hip_matrix_create": (C.c_int32, [C.c_void_p, vpp]),
+    "lamb_hip_matrix_destroy": (C.c_int32, [C.c_void_p]),
+    "lamb_hip_matrix_copy": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "lamb_hip_sector_upload": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_int64, i64p,
+                                           i16p, i64p, f64p, C.c_int64]),
+    "lamb_hip_distribute": (C.c_int32, [C.c_void_p, C.c_int64, i32p, i16p, i64p, f64p,
+                                        C.c_int64]),
+    "lamb_hip_matrix_commit": (C.c_int32, [C.c_void_p]),
+    "lamb_hip_sector_query": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, i64p, i64p]),
+    "lamb_hip_sector_download": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, i64p, i16p,
+                                             i64p, f64p]),
+    "lamb_hip_matrix_info": (C.c_int32, [C.c_void_p, i64p, i64p]),
+    "lamb_hip_multiply": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_double, C.c_double, C.c_double, C.c_int32,
+                                      C.POINTER(Stats)]),
+    "lamb_hip_trace": (C.c_int32, [C.c_void_p, f64p]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load the HIP library, raising if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m lamb_amd.build` "
+            "(the HIP library is the only compute path)")
+    lib = C.CDLL(os.fspath(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def ptr(a, ctype):
+    """Pointer to a contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ctype))

@@ -1,0 +1,84 @@
+// Internal declarations shared by the C-ABI layer and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/lamb_hip.h"
+
+namespace lamb {
+
+// Tile geometry of the numeric kernel: C tiles are (row band) x (col band),
+// a band being up to kBandRows rows made of whole atoms, at most
+// kBandAtoms atoms.
+constexpr int kBandRows = 64;
+constexpr int kBandAtoms = 16;
+// K columns staged in LDS per chunk of the numeric kernel.
+constexpr int kChunkK = 32;
+// LDS row pitch (doubles) of the staged panels; +1 breaks bank aliasing.
+constexpr int kPitch = kBandRows + 1;
+
+// One (tile, k) strip pair of the numeric kernel. rmask / cmask select the
+// atoms of the row / column band whose blocks take part (bit = position
+// of the atom inside its band).
+struct Entry {
+  int32_t a_strip;  // index into the A strips
+  int32_t b_strip;  // index into the B strips
+  int32_t kstart;   // first K column of this entry within the tile
+  uint16_t rmask;
+  uint16_t cmask;
+};
+static_assert(sizeof(Entry) == 16, "Entry layout");
+
+struct TileDesc {
+  int32_t I, J;       // row band, col band
+  int32_t e0, ne;     // entries [e0, e0+ne)
+  int32_t ktot;       // sum of k_k over the entries
+  int32_t pad;
+  int64_t out_base;   // element offset of this tile's region in the C pool
+  uint16_t pattern[kBandAtoms];  // structural C pattern: bit j of row i
+};
+static_assert(sizeof(TileDesc) == 64, "TileDesc layout");
+
+// Strips: the blocks of one band restricted to one atom k.
+// A strip (I, k): blocks A_ik for atoms i of row band I.
+// B strip (k, J): blocks B_kj for atoms j of col band J.
+// off[pos] is the element offset of the block at band position pos.
+struct Strip {
+  int32_t k;       // internal atom index of the shared K atom
+  int32_t band;    // I (A strips) or J (B strips)
+  uint16_t mask;   // positions present
+  uint16_t pad0;
+  int32_t pad1;
+  double min_nsq;  // smallest nsq over the present blocks
+  int64_t off[kBandAtoms];
+  double nsq[kBandAtoms];
+};
+
+struct Band {
+  int32_t first;  // first internal atom
+  int32_t natoms;
+  int16_t rs[kBandAtoms + 1];  // row start per position; rs[natoms] = rows
+  int16_t pad[7];
+};
+static_assert(sizeof(Band) == 56, "Band layout");
+
+// Kernel launchers (kernels.hip).
+hipError_t launch_block_nsq(hipStream_t s, int64_t nblk, const int32_t* d_rows,
+                            const int32_t* d_cols, const int32_t* d_blk, const int64_t* d_off,
+                            const double* d_data, double* d_nsq);
+hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* tiles,
+                              const Entry* entries, const Strip* a_strips,
+                              const Strip* b_strips, const Band* bands, const int32_t* d_blk,
+                              const double* a_data, const double* b_data, double alpha,
+                              double beta, double eps2, const int64_t* old_row_p,
+                              const int32_t* old_col, const int64_t* old_off,
+                              const double* old_data, double* c_data, uint16_t* kept);
+hipError_t launch_trace(hipStream_t s, int32_t n_atoms, const int64_t* row_p,
+                        const int32_t* col, const int64_t* off, const int32_t* blk,
+                        const double* data, double* out);
+
+}  // namespace lamb

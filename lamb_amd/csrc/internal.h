@@ -1,0 +1,97 @@
+// Host-side objects behind the opaque C-ABI handles.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+namespace lamb {
+
+// Device allocation that only grows.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t want);
+  hipError_t upload(hipStream_t s, const void* src, size_t n);
+  void release();
+};
+
+struct StagedSector {
+  int32_t sr = 0, sc = 0;
+  std::vector<int64_t> row_p;
+  std::vector<int16_t> col_i;
+  std::vector<int64_t> ext;
+  std::vector<double> data;
+};
+
+struct StagedList {
+  std::vector<int32_t> subset_idx;
+  std::vector<int16_t> local_idx;
+  std::vector<int64_t> ext;
+  std::vector<double> data;
+};
+
+int32_t set_error(lamb_hip_ctx* ctx, int32_t level, int32_t type, const char* fmt, ...);
+int32_t hip_error(lamb_hip_ctx* ctx, hipError_t e, const char* what);
+
+// Recursive coordinate bisection order of n atoms (coords: xyz per atom).
+void spatial_order(int32_t n, const float* coords, std::vector<int32_t>& order);
+
+}  // namespace lamb
+
+struct lamb_hip_ctx {
+  int32_t device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[8] = {};
+  int32_t level = 0;
+  int32_t type = 0;
+  std::string msg;
+  // scratch reused across multiplies
+  lamb::DevBuf d_tiles, d_entries, d_astrips, d_bstrips, d_kept;
+};
+
+struct lamb_hip_dist {
+  lamb_hip_ctx* ctx = nullptr;
+  int32_t n = 0;
+  int32_t n_subsets = 0;
+  int32_t max_blk = 0;
+  std::vector<int32_t> blk;        // by global atom
+  std::vector<int32_t> subset_of;  // 1-based, by global atom
+  std::vector<int32_t> local_of;   // 1-based, by global atom
+  std::vector<std::vector<int32_t>> subset_atoms;  // [subset][local-1] -> global atom
+  std::vector<int32_t> perm;       // global -> internal
+  std::vector<int32_t> iperm;      // internal -> global
+  std::vector<int32_t> blk_int;    // by internal atom
+  std::vector<lamb::Band> bands;
+  std::vector<int32_t> band_of;      // by internal atom
+  std::vector<int32_t> pos_in_band;  // by internal atom
+  lamb::DevBuf d_blk;    // int32 by internal atom
+  lamb::DevBuf d_bands;  // Band[]
+};
+
+struct lamb_hip_mat {
+  lamb_hip_dist* dist = nullptr;
+  // internal block-CSR (rows/cols are internal atom indices, cols ascending)
+  std::vector<int64_t> row_p;
+  std::vector<int32_t> col;
+  std::vector<int64_t> off;
+  std::vector<int32_t> rows;
+  int64_t nblk = 0;
+  int64_t ndata = 0;   // size of the data pool (may hold unused gaps)
+  int64_t nelems = 0;  // values in stored blocks
+  lamb::DevBuf d_data, d_row_p, d_col, d_off, d_rows, d_nsq;
+  std::vector<double> nsq;  // host copy of the block norms
+  bool nsq_valid = false;
+  std::vector<double> host_data;
+  bool host_data_valid = false;
+  std::vector<lamb::StagedSector> staged_sectors;
+  std::vector<lamb::StagedList> staged_lists;
+
+  hipError_t upload_index(hipStream_t s);
+  void release_device();
+  void invalidate_derived();
+};

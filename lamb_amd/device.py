@@ -1,0 +1,157 @@
+"""Device handles over the C-ABI: context, distribution, matrix."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from .matrix import Distribution, LambMatrix, Sector
+
+# error levels and types of the reference (src/lamb_error.F90:116-159)
+LEVELS = {0: "ok", 1: "note", 2: "warning", 3: "failure", 4: "memory", 5: "fatal"}
+FP64 = 0
+BF16_SPLIT = 1
+
+
+class LambError(RuntimeError):
+    def __init__(self, level: int, type_: int, msg: str):
+        super().__init__(f"[{LEVELS.get(level, level)} / type {type_}] {msg}")
+        self.level = level
+        self.type = type_
+        self.msg = msg
+
+
+class Context:
+    """One HIP device + stream (lamb_hip_init)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = N.load()
+        h = C.c_void_p()
+        rc = self.lib.lamb_hip_init(device, C.byref(h))
+        if rc != 0:
+            raise LambError(rc, -2000, f"lamb_hip_init(device={device}) failed")
+        self.h = h
+        self.device = device
+
+    def check(self, rc: int, what: str = ""):
+        if rc == 0:
+            return
+        lvl, typ = C.c_int32(), C.c_int32()
+        buf = C.create_string_buffer(512)
+        self.lib.lamb_hip_last_error(self.h, C.byref(lvl), C.byref(typ), buf, 512)
+        raise LambError(rc, typ.value, f"{what}: {buf.value.decode(errors='replace')}")
+
+    def close(self):
+        if self.h:
+            self.lib.lamb_hip_finalize(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceDistribution:
+    def __init__(self, ctx: Context, dist: Distribution, spatial: bool = True):
+        self.ctx = ctx
+        self.dist = dist
+        blk = np.ascontiguousarray(dist.blk.astype(np.int8))
+        sm = np.ascontiguousarray(dist.subset_map.astype(np.int32))
+        om = np.ascontiguousarray(dist.order_map.astype(np.int16))
+        xyz = np.ascontiguousarray(dist.coords.astype(np.float32).reshape(-1)) if spatial else None
+        box = np.full(3, dist.box, np.float32)
+        h = C.c_void_p()
+        rc = ctx.lib.lamb_hip_dist_create(ctx.h, dist.n_atoms, N.ptr(blk, C.c_int8),
+                                          dist.n_subsets, N.ptr(sm, C.c_int32),
+                                          N.ptr(om, C.c_int16), N.ptr(xyz, C.c_float),
+                                          N.ptr(box, C.c_float), C.byref(h))
+        ctx.check(rc, "dist_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.lamb_hip_dist_destroy(self.h)
+            self.h = None
+
+
+class DeviceMatrix:
+    """A device-resident block matrix (lamb_hip_mat)."""
+
+    def __init__(self, ddist: DeviceDistribution):
+        self.ddist = ddist
+        self.ctx = ddist.ctx
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.lib.lamb_hip_matrix_create(ddist.h, C.byref(h)), "matrix_create")
+        self.h = h
+
+    @classmethod
+    def from_lamb(cls, ddist: DeviceDistribution, m: LambMatrix) -> "DeviceMatrix":
+        dm = cls(ddist)
+        dm.upload(m)
+        return dm
+
+    def upload(self, m: LambMatrix):
+        lib = self.ctx.lib
+        for sec in m.sectors:
+            r, s = sec.subsets
+            row_p = np.ascontiguousarray(sec.row_p, dtype=np.int64)
+            col_i = np.ascontiguousarray(sec.col_i, dtype=np.int16)
+            ext = np.ascontiguousarray(sec.extents, dtype=np.int64).reshape(-1)
+            data = np.ascontiguousarray(sec.data, dtype=np.float64)
+            rc = lib.lamb_hip_sector_upload(self.h, r, s, sec.n_blks, N.ptr(row_p, C.c_int64),
+                                            N.ptr(col_i, C.c_int16), N.ptr(ext, C.c_int64),
+                                            N.ptr(data, C.c_double), len(data))
+            self.ctx.check(rc, f"sector_upload({r},{s})")
+        self.ctx.check(lib.lamb_hip_matrix_commit(self.h), "matrix_commit")
+
+    def download(self) -> LambMatrix:
+        lib = self.ctx.lib
+        dist = self.ddist.dist
+        sectors = []
+        for (r, s) in dist.sectors:
+            nb, nd = C.c_int64(), C.c_int64()
+            self.ctx.check(lib.lamb_hip_sector_query(self.h, r, s, C.byref(nb), C.byref(nd)),
+                           "sector_query")
+            n_rows = len(dist.subset_atoms[r - 1])
+            row_p = np.zeros(n_rows + 1, np.int64)
+            col_i = np.zeros(nb.value, np.int16)
+            ext = np.zeros((nb.value, 2), np.int64)
+            data = np.zeros(nd.value, np.float64)
+            rc = lib.lamb_hip_sector_download(self.h, r, s, N.ptr(row_p, C.c_int64),
+                                              N.ptr(col_i, C.c_int16), N.ptr(ext, C.c_int64),
+                                              N.ptr(data, C.c_double))
+            self.ctx.check(rc, f"sector_download({r},{s})")
+            sectors.append(Sector((r, s), row_p, col_i, ext, data))
+        return LambMatrix(dist, sectors)
+
+    def copy_from(self, other: "DeviceMatrix"):
+        self.ctx.check(self.ctx.lib.lamb_hip_matrix_copy(other.h, self.h), "matrix_copy")
+
+    def info(self):
+        nb, ne = C.c_int64(), C.c_int64()
+        self.ctx.check(self.ctx.lib.lamb_hip_matrix_info(self.h, C.byref(nb), C.byref(ne)), "info")
+        return nb.value, ne.value
+
+    def trace(self) -> float:
+        t = C.c_double()
+        self.ctx.check(self.ctx.lib.lamb_hip_trace(self.h, C.byref(t)), "trace")
+        return t.value
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.lamb_hip_matrix_destroy(self.h)
+            self.h = None
+
+
+def multiply(a: DeviceMatrix, b: DeviceMatrix, c: DeviceMatrix, *, alpha: float = 1.0,
+             beta: float = 0.0, eps_filter: float = 0.0, flags: int = FP64) -> dict:
+    """C = alpha*A*B + beta*C on the device; returns the call's statistics."""
+    ctx = a.ctx
+    st = N.Stats()
+    rc = ctx.lib.lamb_hip_multiply(ctx.h, a.h, b.h, c.h, float(alpha), float(beta),
+                                   float(eps_filter), int(flags), C.byref(st))
+    ctx.check(rc, "multiply")
+    return st.as_dict()

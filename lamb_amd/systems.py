@@ -1,0 +1,244 @@
+"""Seeded synthetic atom-box systems and their block-sparse matrices.
+
+Atoms sit in a periodic cubic box. Block (i, j) of a matrix is stored iff
+the minimum-image distance r_ij is below a cutoff. Block (i, j) is an
+m_i x n_j FP64 array, column-major, the DBCSR convention the reference's
+loader uses (extent length = row_blk_size * col_blk_size,
+src/lamb_io_operations.F90:680-682). Atom coordinates are generated in FP64,
+rounded to FP32 (the reference's ``space_k = real_4``,
+src/lamb_kinds.F90:179) and wrapped into [0, L); distances are then taken
+in FP64 on those FP32 values, so every consumer sees one pattern.
+
+The matrix returned here is the *global* block-CSR in atom (arrival) order:
+``row_p`` int64[n+1] (0-based exclusive offsets, as the reference's
+``row_p``, src/lamb_index_methods.F90:248-274), ``col`` int32 (0-based),
+``blk_off`` int64 (0-based element offset of each block in ``data``) and
+``data`` float64. The reference's sector split is applied on top of it by
+:mod:`lamb_amd.matrix`.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+O_BLOCK = 13
+H_BLOCK = 5
+X_BLOCK = 23
+
+
+@dataclasses.dataclass
+class AtomBox:
+    """Atoms in a periodic cube: FP32 wrapped positions and block sizes."""
+
+    pos: np.ndarray  # float32 [n, 3], wrapped to [0, L)
+    blk: np.ndarray  # int32 [n] block size per atom
+    box: float       # cube edge L in Angstrom
+
+    @property
+    def n_atoms(self) -> int:
+        return int(self.blk.shape[0])
+
+
+@dataclasses.dataclass
+class BlockCSR:
+    """Global block-CSR of a square atom-blocked matrix (atom arrival order)."""
+
+    blk: np.ndarray      # int32 [n] block size per atom (rows == cols)
+    row_p: np.ndarray    # int64 [n+1]
+    col: np.ndarray      # int32 [nblk]
+    blk_off: np.ndarray  # int64 [nblk]
+    data: np.ndarray     # float64 [sum m_i n_j]
+    dist: np.ndarray | None = None  # float64 [nblk] atom distance per block
+
+    @property
+    def n_atoms(self) -> int:
+        return int(self.blk.shape[0])
+
+    @property
+    def n_blocks(self) -> int:
+        return int(self.col.shape[0])
+
+    def rows(self) -> np.ndarray:
+        return np.repeat(np.arange(self.n_atoms, dtype=np.int32),
+                         np.diff(self.row_p).astype(np.int64))
+
+    def to_dense(self) -> np.ndarray:
+        """Dense FP64 copy (small systems only)."""
+        offs = np.concatenate([[0], np.cumsum(self.blk)]).astype(np.int64)
+        n = int(offs[-1])
+        out = np.zeros((n, n), dtype=np.float64)
+        rows = self.rows()
+        for b in range(self.n_blocks):
+            i, j = int(rows[b]), int(self.col[b])
+            m, k = int(self.blk[i]), int(self.blk[j])
+            o = int(self.blk_off[b])
+            out[offs[i]:offs[i] + m, offs[j]:offs[j] + k] = \
+                self.data[o:o + m * k].reshape(k, m).T
+        return out
+
+    def block(self, b: int) -> np.ndarray:
+        rows = self.rows()
+        i, j = int(rows[b]), int(self.col[b])
+        m, k = int(self.blk[i]), int(self.blk[j])
+        o = int(self.blk_off[b])
+        return self.data[o:o + m * k].reshape(k, m).T
+
+
+def _wrap32(x: np.ndarray, box: float) -> np.ndarray:
+    """Round FP64 coordinates to FP32 and wrap them into [0, L)."""
+    p = np.mod(x, box).astype(np.float32)
+    L32 = np.float32(box)
+    p = np.where(p >= L32, p - L32, p)
+    p = np.where(p < 0, p + L32, p).astype(np.float32)
+    return p
+
+
+def uniform_box(n_atoms: int, box: float, block: int, seed: int) -> AtomBox:
+    """One kind of atom at uniform-random positions."""
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0.0, box, size=(n_atoms, 3))
+    return AtomBox(_wrap32(x, box), np.full(n_atoms, block, np.int32), box)
+
+
+def water_box(n_molecules: int, box: float, seed: int,
+              heavy_fraction: float = 0.0) -> AtomBox:
+    """Water-like box: O at random positions, each H at 0.96 A from its O.
+
+    Atoms are listed molecule by molecule (O, H, H). With
+    ``heavy_fraction`` > 0 that fraction of all atoms (taken from the O
+    sites, seeded) becomes a third kind with 23 x 23 blocks.
+    """
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(0.0, box, size=(n_molecules, 3))
+    v = rng.normal(size=(n_molecules, 2, 3))
+    v /= np.linalg.norm(v, axis=2, keepdims=True)
+    h = o[:, None, :] + 0.96 * v
+    x = np.concatenate([o[:, None, :], h], axis=1).reshape(-1, 3)
+    blk = np.tile(np.array([O_BLOCK, H_BLOCK, H_BLOCK], np.int32), n_molecules)
+    if heavy_fraction > 0.0:
+        n_heavy = min(n_molecules, int(round(heavy_fraction * 3 * n_molecules)))
+        sel = rng.choice(n_molecules, size=n_heavy, replace=False)
+        blk[3 * sel] = X_BLOCK
+    return AtomBox(_wrap32(x, box), blk, box)
+
+
+def neighbour_pairs(atoms: AtomBox, cutoff: float):
+    """All (i, j, r) with minimum-image distance r < cutoff, CSR-ordered.
+
+    Distances are FP64 on the FP32 positions. Returns int32 i, int32 j,
+    float64 r sorted by (i, j); the diagonal (r = 0) is included.
+    """
+    pos = atoms.pos.astype(np.float64)
+    L = float(atoms.box)
+    n = pos.shape[0]
+    ii, jj, rr = [], [], []
+    chunk = max(1, min(n, (1 << 22) // max(n, 1)))
+    c2 = cutoff * cutoff
+    for s in range(0, n, chunk):
+        d = pos[s:s + chunk, None, :] - pos[None, :, :]
+        d -= L * np.round(d / L)
+        r2 = np.einsum("abk,abk->ab", d, d)
+        a, b = np.nonzero(r2 < c2)
+        ii.append((a + s).astype(np.int32))
+        jj.append(b.astype(np.int32))
+        rr.append(np.sqrt(r2[a, b]))
+    return np.concatenate(ii), np.concatenate(jj), np.concatenate(rr)
+
+
+def _offsets(sizes: np.ndarray) -> np.ndarray:
+    return np.concatenate([[0], np.cumsum(sizes.astype(np.int64))]).astype(np.int64)
+
+
+def decay_matrix(atoms: AtomBox, cutoff: float, decay: float, seed: int,
+                 symmetric: bool = False, diag_shift: float = 0.0) -> BlockCSR:
+    """Block-sparse matrix with entries U[-1,1] * exp(-decay * r_ij).
+
+    Blocks are generated in CSR order. With ``symmetric`` the upper blocks
+    (i < j) are drawn and mirrored (block (j,i) = block(i,j)^T), diagonal
+    blocks are symmetrised as (X + X^T)/2 and ``diag_shift`` * I is added.
+    """
+    i, j, r = neighbour_pairs(atoms, cutoff)
+    blk = atoms.blk
+    n = atoms.n_atoms
+    m = blk[i].astype(np.int64)
+    k = blk[j].astype(np.int64)
+    sizes = m * k
+    blk_off = _offsets(sizes)[:-1]
+    total = int(sizes.sum())
+    rng = np.random.default_rng(seed)
+    row_p = np.zeros(n + 1, np.int64)
+    np.add.at(row_p, i.astype(np.int64) + 1, 1)
+    row_p = np.cumsum(row_p)
+    if not symmetric:
+        vals = rng.uniform(-1.0, 1.0, size=total)
+        scale = np.repeat(np.exp(-decay * r), sizes)
+        data = vals * scale
+    else:
+        data = np.empty(total, np.float64)
+        upper = i <= j
+        u_sizes = sizes[upper]
+        vals = rng.uniform(-1.0, 1.0, size=int(u_sizes.sum()))
+        vals *= np.repeat(np.exp(-decay * r[upper]), u_sizes)
+        u_off = _offsets(u_sizes)
+        # position of block (j, i) for every (i, j): lookup by key
+        key = i.astype(np.int64) * n + j
+        tkey = j.astype(np.int64) * n + i
+        order = np.argsort(key)
+        tpos = order[np.searchsorted(key[order], tkey)]
+        uidx = np.flatnonzero(upper)
+        for t, b in enumerate(uidx):
+            mm, kk = int(m[b]), int(k[b])
+            x = vals[u_off[t]:u_off[t + 1]].reshape(kk, mm).T  # m x k
+            if i[b] == j[b]:
+                x = 0.5 * (x + x.T) + diag_shift * np.eye(mm)
+                data[blk_off[b]:blk_off[b] + mm * kk] = x.T.reshape(-1)
+            else:
+                data[blk_off[b]:blk_off[b] + mm * kk] = x.T.reshape(-1)
+                tb = tpos[b]
+                data[blk_off[tb]:blk_off[tb] + mm * kk] = x.reshape(-1)
+    return BlockCSR(blk.astype(np.int32), row_p, j.astype(np.int32),
+                    blk_off.astype(np.int64), data, r)
+
+
+# --- the named workloads --------------------------------------------------
+
+def config1(seed: int = 1):
+    """512 atoms, U[0,17.4)^3, block 13, r < 6 A, exp(-0.5 r), A = B symmetric
+    with diagonal blocks symmetrised + 2I; eps_filter 1e-8."""
+    atoms = uniform_box(512, 17.4, O_BLOCK, seed)
+    a = decay_matrix(atoms, 6.0, 0.5, seed + 1000, symmetric=True, diag_shift=2.0)
+    return atoms, a, a, 1e-8
+
+
+def config2(seed: int = 2):
+    """2,048 H2O (6,144 atoms), 39.4 A box, O 13 / H 5, r < 8 A, exp(-0.4 r),
+    A and B from independent value seeds; eps_filter 1e-7."""
+    atoms = water_box(2048, 39.4, seed)
+    a = decay_matrix(atoms, 8.0, 0.4, seed + 1000)
+    b = decay_matrix(atoms, 8.0, 0.4, seed + 2000)
+    return atoms, a, b, 1e-7
+
+
+def water_system(n_atoms: int, seed: int, cutoff: float = 8.0,
+                 heavy_fraction: float = 0.0, density: float = 0.1):
+    """Water-like box of about ``n_atoms`` atoms at ``density`` atoms/A^3."""
+    n_mol = max(1, int(round(n_atoms / 3)))
+    box = float((3 * n_mol / density) ** (1.0 / 3.0))
+    atoms = water_box(n_mol, box, seed, heavy_fraction)
+    return atoms
+
+
+def small_system(n_atoms: int, seed: int, box: float | None = None,
+                 sizes=(5, 13), cutoff: float = 6.0, decay: float = 0.5,
+                 symmetric: bool = False):
+    """Tiny mixed-size test system (random kinds), for brute-force checks."""
+    rng = np.random.default_rng(seed)
+    if box is None:
+        box = float((n_atoms / 0.1) ** (1.0 / 3.0))
+    x = rng.uniform(0.0, box, size=(n_atoms, 3))
+    blk = rng.choice(np.asarray(sizes, np.int32), size=n_atoms).astype(np.int32)
+    atoms = AtomBox(_wrap32(x, box), blk, box)
+    m = decay_matrix(atoms, cutoff, decay, seed + 7, symmetric=symmetric,
+                     diag_shift=2.0 if symmetric else 0.0)
+    return atoms, m

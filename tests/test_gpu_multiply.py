@@ -1,0 +1,158 @@
+"""GPU parity: the HIP multiply through the C-ABI against the CPU oracle.
+
+Bar: identical stored block pattern (row_p, col) and kept-triple counts;
+values within 1e-12 relative Frobenius error (FP64, different but fixed
+summation order); integer-valued and identity cases bit-exact.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from __graft_entry__ import compare
+from lamb_amd import device as D
+from lamb_amd import systems as S
+from lamb_amd.matrix import Distribution, LambMatrix
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def gpu_multiply(ctx, atoms, a, b, *, eps=0.0, alpha=1.0, beta=0.0, c_old=None, spatial=True):
+    dist = Distribution(atoms.blk, atoms.pos, atoms.box)
+    dd = D.DeviceDistribution(ctx, dist, spatial=spatial)
+    da = D.DeviceMatrix.from_lamb(dd, LambMatrix.from_global(dist, a))
+    db = D.DeviceMatrix.from_lamb(dd, LambMatrix.from_global(dist, b))
+    dc = D.DeviceMatrix(dd)
+    if c_old is not None:
+        dc.upload(LambMatrix.from_global(dist, c_old))
+    st = D.multiply(da, db, dc, alpha=alpha, beta=beta, eps_filter=eps)
+    got = dc.download().to_global()
+    for h in (dc, db, da, dd):
+        h.close()
+    return got, st
+
+
+def check(ctx, atoms, a, b, *, eps=0.0, alpha=1.0, beta=0.0, c_old=None, spatial=True,
+          exact=False):
+    print(f"case: atoms={atoms.n_atoms} blocks={a.n_blocks} eps={eps} alpha={alpha} beta={beta}",
+          flush=True)
+    got, st = gpu_multiply(ctx, atoms, a, b, eps=eps, alpha=alpha, beta=beta, c_old=c_old,
+                           spatial=spatial)
+    ref, rst = oracle.multiply(a, b, eps=eps, alpha=alpha, beta=beta, c_old=c_old)
+    same, err = compare(got, ref)
+    assert same, "stored block pattern differs from the oracle"
+    if exact:
+        assert np.array_equal(got.data, ref.data)
+    else:
+        assert err <= TOL, err
+    assert st["n_triples_kept"] == rst["n_triples_kept"]
+    assert st["n_triples_filtered"] == rst["n_triples_filtered"]
+    assert st["flops"] == rst["flops"]
+    assert st["n_c_blocks_kept"] == ref.n_blocks
+    return st, rst
+
+
+def test_tiny_mixed(ctx):
+    atoms, a = S.small_system(20, 1, sizes=(5, 13))
+    b = S.decay_matrix(atoms, 6.0, 0.5, 2)
+    check(ctx, atoms, a, b)
+
+
+def test_water_eps(ctx):
+    atoms = S.water_system(96, seed=11)
+    a = S.decay_matrix(atoms, 8.0, 0.4, 12)
+    b = S.decay_matrix(atoms, 8.0, 0.4, 13)
+    check(ctx, atoms, a, b, eps=1e-7)
+
+
+def test_three_kinds(ctx):
+    atoms = S.water_system(300, seed=21, heavy_fraction=0.1)
+    assert set(np.unique(atoms.blk)) == {5, 13, 23}
+    a = S.decay_matrix(atoms, 8.0, 0.4, 22)
+    b = S.decay_matrix(atoms, 8.0, 0.4, 23)
+    check(ctx, atoms, a, b, eps=1e-7)
+
+
+def test_config1_symmetric(ctx):
+    atoms, a, b, eps = S.config1()
+    st, rst = check(ctx, atoms, a, b, eps=eps)
+    assert st["n_triples_filtered"] == 0
+
+
+def test_filters_bind(ctx):
+    atoms, a = S.small_system(60, 31, cutoff=9.0, decay=1.5, sizes=(1, 5, 13))
+    b = S.decay_matrix(atoms, 9.0, 1.3, 32)
+    st, rst = check(ctx, atoms, a, b, eps=2e-3)
+    assert rst["n_triples_filtered"] > 0
+    assert rst["n_struct_blocks"] > st["n_c_blocks_kept"]  # the post-filter dropped blocks
+
+
+def test_alpha_beta(ctx):
+    atoms = S.water_system(150, seed=41)
+    a = S.decay_matrix(atoms, 6.0, 0.4, 42)
+    b = S.decay_matrix(atoms, 6.0, 0.4, 43)
+    old = S.decay_matrix(atoms, 9.0, 0.4, 44)
+    check(ctx, atoms, a, b, eps=1e-9, alpha=-2.0, beta=3.0, c_old=old)
+
+
+def test_arrival_order_without_coordinates(ctx):
+    atoms = S.water_system(120, seed=51)
+    a = S.decay_matrix(atoms, 8.0, 0.4, 52)
+    b = S.decay_matrix(atoms, 8.0, 0.4, 53)
+    check(ctx, atoms, a, b, eps=1e-7, spatial=False)
+
+
+def test_identity_is_exact(ctx):
+    atoms, a = S.small_system(30, 61, sizes=(5, 13, 23))
+    n = a.n_atoms
+    off = np.concatenate([[0], np.cumsum(a.blk.astype(np.int64) ** 2)[:-1]])
+    data = np.concatenate([np.eye(int(m)).reshape(-1, order="F") for m in a.blk])
+    eye = S.BlockCSR(a.blk.copy(), np.arange(n + 1, dtype=np.int64),
+                     np.arange(n, dtype=np.int32), off.astype(np.int64), data)
+    check(ctx, atoms, eye, a, exact=True)
+    check(ctx, atoms, a, eye, exact=True)
+
+
+def test_integer_golden_is_exact(ctx):
+    fx = np.load("tests/golden/integer_product.npz")
+    blk = fx["blk"]
+    rng = np.random.default_rng(0)
+    atoms = S.AtomBox(rng.uniform(0, 10, (len(blk), 3)).astype(np.float32), blk, 10.0)
+    a = S.BlockCSR(blk, fx["a_row_p"], fx["a_col"], fx["a_off"], fx["a_data"])
+    b = S.BlockCSR(blk, fx["b_row_p"], fx["b_col"], fx["b_off"], fx["b_data"])
+    got, st = gpu_multiply(ctx, atoms, a, b, eps=float(fx["eps"]))
+    assert np.array_equal(got.row_p, fx["c_row_p"]) and np.array_equal(got.col, fx["c_col"])
+    assert np.array_equal(got.data, fx["c_data"])
+    assert st["n_triples_kept"] == int(fx["kept"])
+
+
+def test_empty_operand(ctx):
+    atoms, a = S.small_system(12, 71)
+    empty = S.BlockCSR(a.blk.copy(), np.zeros(a.n_atoms + 1, np.int64), np.zeros(0, np.int32),
+                       np.zeros(0, np.int64), np.zeros(0))
+    got, st = gpu_multiply(ctx, atoms, empty, a)
+    assert got.n_blocks == 0 and st["n_triples_kept"] == 0
+
+
+def test_zero_size_blocks(ctx):
+    atoms, a = S.small_system(25, 81, sizes=(0, 2, 3))
+    b = S.decay_matrix(atoms, 6.0, 0.5, 82)
+    check(ctx, atoms, a, b, eps=0.0)
+    check(ctx, atoms, a, b, eps=1e-3)
+
+
+def test_upload_download_round_trip(ctx):
+    atoms = S.water_system(200, seed=91, heavy_fraction=0.1)
+    a = S.decay_matrix(atoms, 8.0, 0.4, 92)
+    dist = Distribution(atoms.blk, atoms.pos, atoms.box)
+    lm = LambMatrix.from_global(dist, a)
+    dd = D.DeviceDistribution(ctx, dist)
+    dm = D.DeviceMatrix.from_lamb(dd, lm)
+    back = dm.download()
+    for s0, s1 in zip(lm.sectors, back.sectors):
+        assert s0.subsets == s1.subsets
+        assert np.array_equal(s0.row_p, s1.row_p) and np.array_equal(s0.col_i, s1.col_i)
+        assert np.array_equal(s0.extents, s1.extents) and np.array_equal(s0.data, s1.data)
+    assert dm.trace() == pytest.approx(oracle.trace(a), rel=1e-14, abs=1e-14)
+    dm.close()
+    dd.close()
