@@ -17,21 +17,25 @@ namespace lamb {
 constexpr int kBandRows = 64;
 constexpr int kBandAtoms = 16;
 // K columns staged in LDS per chunk of the numeric kernel.
-constexpr int kChunkK = 32;
+constexpr int kChunkK = 16;
 // LDS row pitch (doubles) of the staged panels; +1 breaks bank aliasing.
 constexpr int kPitch = kBandRows + 1;
 
-// One (tile, k) strip pair of the numeric kernel. rmask / cmask select the
-// atoms of the row / column band whose blocks take part (bit = position
-// of the atom inside its band).
+// One (tile, k) strip pair of the numeric kernel. a_src / b_src are the
+// element offsets of the packed A strip [kdim][64] and packed B strip
+// [kdim][64]; rmask / cmask select the atoms of the row / column band whose
+// blocks take part (bit = position of the atom inside its band).
 struct Entry {
-  int32_t a_strip;  // index into the A strips
-  int32_t b_strip;  // index into the B strips
+  int64_t a_src;
+  int64_t b_src;
   int32_t kstart;   // first K column of this entry within the tile
+  int16_t kdim;     // K columns of this entry (block size of atom k)
+  int16_t pad0;
   uint16_t rmask;
   uint16_t cmask;
+  int32_t pad1;
 };
-static_assert(sizeof(Entry) == 16, "Entry layout");
+static_assert(sizeof(Entry) == 32, "Entry layout");
 
 struct TileDesc {
   int32_t I, J;       // row band, col band
@@ -67,13 +71,17 @@ struct Band {
 static_assert(sizeof(Band) == 56, "Band layout");
 
 // Kernel launchers (kernels.hip).
+// Pack strips into dense [kdim][64] panels: A strips put band rows along the
+// 64 (row_side = true), B strips put band columns along the 64.
+hipError_t launch_pack_strips(hipStream_t s, int64_t n_strips, const Strip* strips,
+                              const int64_t* pk_off, const Band* bands, const int32_t* d_blk,
+                              const double* data, bool row_side, double* packed);
 hipError_t launch_block_nsq(hipStream_t s, int64_t nblk, const int32_t* d_rows,
                             const int32_t* d_cols, const int32_t* d_blk, const int64_t* d_off,
                             const double* d_data, double* d_nsq);
 hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* tiles,
-                              const Entry* entries, const Strip* a_strips,
-                              const Strip* b_strips, const Band* bands, const int32_t* d_blk,
-                              const double* a_data, const double* b_data, double alpha,
+                              const Entry* entries, const Band* bands,
+                              const double* a_pack, const double* b_pack, double alpha,
                               double beta, double eps2, const int64_t* old_row_p,
                               const int32_t* old_col, const int64_t* old_off,
                               const double* old_data, double* c_data, uint16_t* kept);

@@ -52,6 +52,7 @@ struct lamb_hip_ctx {
   std::string msg;
   // scratch reused across multiplies
   lamb::DevBuf d_tiles, d_entries, d_astrips, d_bstrips, d_kept;
+  lamb::DevBuf d_apk_off, d_bpk_off, d_apack, d_bpack;
 };
 
 struct lamb_hip_dist {

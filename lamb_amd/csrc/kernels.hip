@@ -34,13 +34,69 @@ hipError_t launch_block_nsq(hipStream_t s, int64_t nblk, const int32_t* d_rows,
 }
 
 // ---------------------------------------------------------------------------
+// Strip packing: each strip (the blocks of one band against one atom k)
+// becomes a dense [kdim][64] panel, zero where a band atom has no block.
+// A strips (row_side): panel[kk][r] = A_{i(r),k}(r_in, kk).
+// B strips:            panel[kk][c] = B_{k,j(c)}(kk, c_in).
+// One wave per strip; lane = band row / column.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void pack_strips_kernel(int64_t n_strips,
+                                                          const Strip* __restrict__ strips,
+                                                          const int64_t* __restrict__ pk_off,
+                                                          const Band* __restrict__ bands,
+                                                          const int32_t* __restrict__ blk,
+                                                          const double* __restrict__ data,
+                                                          int row_side, double* __restrict__ packed) {
+  const int lane = threadIdx.x & 63;
+  const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (s >= n_strips) return;
+  const int32_t band = strips[s].band;
+  const int32_t k = strips[s].k;
+  const uint32_t mask = strips[s].mask;
+  const int kdim = blk[k];
+  const int natoms = bands[band].natoms;
+  int pos = -1, lo = 0, hi = 0;
+  for (int a = 0; a < natoms; ++a) {
+    const int r0 = bands[band].rs[a], r1 = bands[band].rs[a + 1];
+    if (lane >= r0 && lane < r1) {
+      pos = a;
+      lo = r0;
+      hi = r1;
+    }
+  }
+  const bool have = pos >= 0 && ((mask >> pos) & 1u);
+  const int64_t o = have ? strips[s].off[pos] : 0;
+  const int in = lane - lo, ext = hi - lo;
+  double* out = packed + pk_off[s];
+  for (int kk = 0; kk < kdim; ++kk) {
+    double v = 0.0;
+    if (have) v = row_side ? data[o + (int64_t)kk * ext + in] : data[o + (int64_t)in * kdim + kk];
+    out[kk * 64 + lane] = v;
+  }
+}
+
+hipError_t launch_pack_strips(hipStream_t s, int64_t n_strips, const Strip* strips,
+                              const int64_t* pk_off, const Band* bands, const int32_t* d_blk,
+                              const double* data, bool row_side, double* packed) {
+  if (n_strips == 0) return hipSuccess;
+  const int64_t grid = (n_strips + 3) / 4;
+  hipLaunchKernelGGL(pack_strips_kernel, dim3((unsigned)grid), dim3(256), 0, s, n_strips, strips,
+                     pk_off, bands, d_blk, data, row_side ? 1 : 0, packed);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // FP64 numeric phase.
 //
 // One workgroup (4 waves) owns one C tile = row band I x col band J, at most
 // 64 x 64 values. The tile's K dimension is the concatenation of the K atoms
-// of its entries; it is staged through LDS kChunkK columns at a time:
-//   As[q][r] = A strip value (tile row r, K column q), zero where absent,
-//   Bs[q][c] = B strip value (K column q, tile col c).
+// of its entries, streamed through LDS kChunkK columns at a time:
+//   As[q][r] = packed A strip value (tile row r, K column q),
+//   Bs[q][c] = packed B strip value (K column q, tile col c).
+// Pipeline per chunk c (one barrier): issue the global loads of chunk c+1
+// into registers, wave 0 builds the column table of chunk c+2, all waves
+// run the MFMAs of chunk c, then the registers go to the other LDS buffer.
+//
 // Each wave owns a 32 x 32 quadrant as 64 4x4 sub-tiles, held as 16
 // accumulators of v_mfma_f64_4x4x4_4b (4 independent 4x4x4 blocks per
 // instruction): accumulator h*8+g holds the 4 row groups of half h (blocks
@@ -52,10 +108,11 @@ hipError_t launch_block_nsq(hipStream_t s, int64_t nblk, const int32_t* d_rows,
 // l = a + 4b + 16c; B[c][t] of block b at l = t + 4b + 16c; D[a][t] of
 // block b at l = 16a + 4b + t.
 // ---------------------------------------------------------------------------
-struct ColInfo {
-  int32_t a_strip, b_strip;
-  int16_t kk, kdim;
-  uint16_t rmask, cmask;
+struct ColSrc {
+  int64_t a, b;  // packed source offsets of this K column (-1: zero column)
+  uint32_t rmask, cmask;
+  uint32_t groups;  // row-group mask | column-group mask << 16
+  uint32_t pad;
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -63,180 +120,212 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-__global__ __launch_bounds__(256) void numeric_f64_kernel(
+__global__ __launch_bounds__(256, 4) void numeric_f64_kernel(
     const TileDesc* __restrict__ tiles, const Entry* __restrict__ entries,
-    const Strip* __restrict__ a_strips, const Strip* __restrict__ b_strips,
-    const Band* __restrict__ bands, const int32_t* __restrict__ blk,
-    const double* __restrict__ a_data, const double* __restrict__ b_data, double alpha,
-    double beta, double eps2, const int64_t* __restrict__ old_row_p,
-    const int32_t* __restrict__ old_col, const int64_t* __restrict__ old_off,
-    const double* __restrict__ old_data, double* __restrict__ c_data,
-    uint16_t* __restrict__ kept_out) {
+    const Band* __restrict__ bands, const double* __restrict__ a_pack,
+    const double* __restrict__ b_pack, double alpha, double beta, double eps2,
+    const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
+    const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
+    double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
   constexpr int KC = kChunkK;
-  __shared__ double lds[2 * KC * kPitch];  // As | Bs, later the C tile [64][kPitch]
-  __shared__ ColInfo colinfo[KC];
-  __shared__ uint32_t smask_r[KC / 4], smask_c[KC / 4];
-  __shared__ int8_t rowpos[kBandRows], rowin[kBandRows], colpos[kBandRows], colin[kBandRows];
+  constexpr int PANEL = KC * kPitch;
+  constexpr int NT = 4;  // column tables in flight (chunks c .. c+3)
+  constexpr int CT = kBandRows * kPitch;
+  __shared__ double lds[(4 * PANEL > CT) ? 4 * PANEL : CT];  // [A0 B0 A1 B1]; epilogue C tile
+  __shared__ ColSrc tab[NT][KC];
+  __shared__ uint32_t smr[NT][KC / 4], smc[NT][KC / 4];
   __shared__ int16_t rs_r[kBandAtoms + 1], rs_c[kBandAtoms + 1];
   __shared__ uint16_t pg_r[kBandAtoms], pg_c[kBandAtoms];
-  __shared__ int32_t e_cursor;
   __shared__ int32_t pair_id[kBandAtoms * kBandAtoms];
   __shared__ int64_t pair_off[kBandAtoms * kBandAtoms];
-  __shared__ int32_t wave_tot[4];
+  __shared__ int32_t wave_cnt[4];
+  __shared__ int64_t wave_sz[4];
   __shared__ uint32_t keptw[kBandAtoms];
-
-  double* As = lds;
-  double* Bs = lds + KC * kPitch;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wave = tid >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wr = wave >> 1, wc = wave & 1;
 
-  const TileDesc T = tiles[blockIdx.x];
-  const Band bI = bands[T.I];
-  const Band bJ = bands[T.J];
+  // Workgroups are dealt round-robin over the 8 XCDs; remap so that each XCD
+  // walks one contiguous range of tiles (shared A strips stay in its L2).
+  const int nb = gridDim.x;
+  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int per = nb >> 3, rem = nb & 7;
+  const int tile = (xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per) + slot;
+  const TileDesc* tp = tiles + tile;
+  const int I = tp->I, J = tp->J, e0 = tp->e0, ne = tp->ne, ktot = tp->ktot;
+  const int64_t out_base = tp->out_base;
+  const int natI = bands[I].natoms, natJ = bands[J].natoms;
+  const int32_t firstI = bands[I].first, firstJ = bands[J].first;
 
   if (tid <= kBandAtoms) {
-    rs_r[tid] = bI.rs[tid < bI.natoms ? tid : bI.natoms];
-    rs_c[tid] = bJ.rs[tid < bJ.natoms ? tid : bJ.natoms];
+    rs_r[tid] = bands[I].rs[min(tid, natI)];
+    rs_c[tid] = bands[J].rs[min(tid, natJ)];
   }
-  if (tid < kBandRows) {
-    int p = -1;
-    for (int a = 0; a < bI.natoms; ++a)
-      if (tid >= bI.rs[a] && tid < bI.rs[a + 1]) p = a;
-    rowpos[tid] = (int8_t)p;
-    rowin[tid] = (int8_t)(p >= 0 ? tid - bI.rs[p] : 0);
-  } else if (tid < 2 * kBandRows) {
-    const int c = tid - kBandRows;
-    int p = -1;
-    for (int a = 0; a < bJ.natoms; ++a)
-      if (c >= bJ.rs[a] && c < bJ.rs[a + 1]) p = a;
-    colpos[c] = (int8_t)p;
-    colin[c] = (int8_t)(p >= 0 ? c - bJ.rs[p] : 0);
-  } else if (tid < 2 * kBandRows + kBandAtoms) {
-    const int p = tid - 2 * kBandRows;
-    uint32_t g = 0;
-    if (p < bI.natoms && bI.rs[p + 1] > bI.rs[p])
-      for (int q = bI.rs[p] / 4; q <= (bI.rs[p + 1] - 1) / 4; ++q) g |= 1u << q;
-    pg_r[p] = (uint16_t)g;
-  } else if (tid < 2 * kBandRows + 2 * kBandAtoms) {
-    const int p = tid - 2 * kBandRows - kBandAtoms;
-    uint32_t g = 0;
-    if (p < bJ.natoms && bJ.rs[p + 1] > bJ.rs[p])
-      for (int q = bJ.rs[p] / 4; q <= (bJ.rs[p + 1] - 1) / 4; ++q) g |= 1u << q;
-    pg_c[p] = (uint16_t)g;
-  }
-  if (tid == 0) e_cursor = T.e0;
   if (tid < kBandAtoms) keptw[tid] = 0;
   __syncthreads();
+  if (tid < kBandAtoms) {
+    uint32_t g = 0;
+    if (tid < natI && rs_r[tid + 1] > rs_r[tid])
+      for (int q = rs_r[tid] / 4; q <= (rs_r[tid + 1] - 1) / 4; ++q) g |= 1u << q;
+    pg_r[tid] = (uint16_t)g;
+    g = 0;
+    if (tid < natJ && rs_c[tid + 1] > rs_c[tid])
+      for (int q = rs_c[tid] / 4; q <= (rs_c[tid + 1] - 1) / 4; ++q) g |= 1u << q;
+    pg_c[tid] = (uint16_t)g;
+  }
+  // band position of this thread's staging row (A) and column (B)
+  int rpos = -1, cpos = -1;
+  for (int a = 0; a < natI; ++a)
+    if (lane >= rs_r[a] && lane < rs_r[a + 1]) rpos = a;
+  for (int a = 0; a < natJ; ++a)
+    if (lane >= rs_c[a] && lane < rs_c[a + 1]) cpos = a;
+  __syncthreads();
+
+  const int nchunks = (ktot + KC - 1) / KC;
+  const int e_end = e0 + ne;
+
+  // Column table of chunk ch into tab[buf] / smr[buf] / smc[buf], built by
+  // wave 0. The entry cursor is a wave-uniform register of wave 0; lanes
+  // exchange data through LDS only across an explicit wavefront fence and
+  // wave barrier.
+  int cursor = e0;
+  auto build_table = [&](int ch, int buf) {
+    const int Q0 = ch * KC;
+    if (lane < KC && Q0 + lane >= ktot) {
+      ColSrc z;
+      z.a = -1;
+      z.b = -1;
+      z.rmask = 0;
+      z.cmask = 0;
+      z.groups = 0;
+      z.pad = 0;
+      tab[buf][lane] = z;
+    }
+    const int e = cursor + lane;
+    const bool valid = lane <= KC && e < e_end;
+    int ks = 0, kd = 0;
+    int64_t asrc = 0, bsrc = 0;
+    uint32_t rm = 0, cm = 0;
+    if (valid) {
+      const Entry* en = entries + e;
+      ks = en->kstart;
+      kd = en->kdim;
+      asrc = en->a_src;
+      bsrc = en->b_src;
+      rm = en->rmask;
+      cm = en->cmask;
+    }
+    const int lo = max(ks, Q0), hi = min(ks + kd, Q0 + KC);
+    if (valid && lo < hi) {
+      uint32_t gr = 0, gc = 0;
+      for (int p = 0; p < kBandAtoms; ++p) {
+        if ((rm >> p) & 1u) gr |= pg_r[p];
+        if ((cm >> p) & 1u) gc |= pg_c[p];
+      }
+      for (int Q = lo; Q < hi; ++Q) {
+        ColSrc cs;
+        cs.a = asrc + (int64_t)(Q - ks) * 64;
+        cs.b = bsrc + (int64_t)(Q - ks) * 64;
+        cs.rmask = rm;
+        cs.cmask = cm;
+        cs.groups = gr | (gc << 16);
+        cs.pad = 0;
+        tab[buf][Q - Q0] = cs;
+      }
+    }
+    const uint64_t done = __ballot(valid && ks + kd <= Q0 + KC);
+    cursor += (int)__popcll(done);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane < KC / 4) {
+      uint32_t g = 0;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) g |= tab[buf][4 * lane + t].groups;
+      smr[buf][lane] = g & 0xFFFFu;
+      smc[buf][lane] = g >> 16;
+    }
+  };
+
+  if (wave == 0)
+    for (int ch = 0; ch < min(nchunks, NT - 1); ++ch) build_table(ch, ch);
+  __syncthreads();
+
+  // Two register sets of staged loads: chunk c+1 and c+2 are in flight while
+  // chunk c runs on the MFMA pipes. Loads are predicated on the entry's atom
+  // masks, so rows / columns of absent atoms cost no memory traffic.
+  constexpr int PER = KC / 4;  // K columns per wave per chunk
+  double ra0[PER], rb0[PER], ra1[PER], rb1[PER];
+#define LAMB_ISSUE(RA, RB, ch)                                             \
+  {                                                                        \
+    const int t_ = (ch) % NT;                                              \
+    _Pragma("unroll") for (int p = 0; p < PER; ++p) {                      \
+      const int q = wave + 4 * p;                                          \
+      const int64_t sa = tab[t_][q].a, sb = tab[t_][q].b;                  \
+      const uint32_t rmk = tab[t_][q].rmask, cmk = tab[t_][q].cmask;       \
+      RA[p] = (sa >= 0 && rpos >= 0 && ((rmk >> rpos) & 1u)) ? a_pack[sa + lane] : 0.0; \
+      RB[p] = (sb >= 0 && cpos >= 0 && ((cmk >> cpos) & 1u)) ? b_pack[sb + lane] : 0.0; \
+    }                                                                      \
+  }
+#define LAMB_COMMIT(RA, RB, ch)                                            \
+  {                                                                        \
+    double* As_ = lds + ((ch) & 1) * 2 * PANEL;                            \
+    double* Bs_ = As_ + PANEL;                                             \
+    _Pragma("unroll") for (int p = 0; p < PER; ++p) {                      \
+      const int q = wave + 4 * p;                                          \
+      As_[q * kPitch + lane] = RA[p];                                      \
+      Bs_[q * kPitch + lane] = RB[p];                                      \
+    }                                                                      \
+  }
+#define LAMB_STEP(RA_NEXT2, RB_NEXT2, RA_NEXT1, RB_NEXT1, c)               \
+  {                                                                        \
+    if ((c) + 2 < nchunks) LAMB_ISSUE(RA_NEXT2, RB_NEXT2, (c) + 2);        \
+    if (wave == 0 && (c) + NT - 1 < nchunks) build_table((c) + NT - 1, ((c) + NT - 1) % NT); \
+    {                                                                      \
+      const double* As = lds + ((c) & 1) * 2 * PANEL;                      \
+      const double* Bs = As + PANEL;                                       \
+      const int t3 = (c) % NT;                                             \
+      const int nsteps = (min(KC, ktot - (c) * KC) + 3) >> 2;              \
+      for (int s = 0; s < nsteps; ++s) {                                   \
+        const uint32_t mr = __builtin_amdgcn_readfirstlane(smr[t3][s]) >> (8 * wr); \
+        const uint32_t mc = __builtin_amdgcn_readfirstlane(smc[t3][s]) >> (8 * wc); \
+        if ((mr & 0xFFu) == 0 || (mc & 0xFFu) == 0) continue;             \
+        const int krow = (4 * s + (lane >> 4)) * kPitch;                   \
+        double bop[8];                                                     \
+        _Pragma("unroll") for (int g = 0; g < 8; ++g)                      \
+          bop[g] = ((mc >> g) & 1u) ? Bs[krow + 32 * wc + 4 * g + (lane & 3)] : 0.0; \
+        _Pragma("unroll") for (int h = 0; h < 2; ++h) {                    \
+          if ((mr >> (4 * h)) & 0xFu) {                                    \
+            const double aop = As[krow + 32 * wr + 16 * h + (lane & 15)];  \
+            _Pragma("unroll") for (int g = 0; g < 8; ++g)                  \
+              if ((mc >> g) & 1u)                                          \
+                acc[h * 8 + g] = __builtin_amdgcn_mfma_f64_4x4x4f64(aop, bop[g], acc[h * 8 + g], 0, 0, 0); \
+          }                                                                \
+        }                                                                  \
+      }                                                                    \
+    }                                                                      \
+    if ((c) + 1 < nchunks) LAMB_COMMIT(RA_NEXT1, RB_NEXT1, (c) + 1);       \
+    __syncthreads();                                                       \
+  }
 
   double acc[16];
 #pragma unroll
   for (int x = 0; x < 16; ++x) acc[x] = 0.0;
 
-  const int nchunks = (T.ktot + KC - 1) / KC;
-  const int e_end = T.e0 + T.ne;
-  for (int ch = 0; ch < nchunks; ++ch) {
-    const int Q0 = ch * KC;
-    if (wave == 0) {
-      const int q = lane;
-      const int Q = Q0 + q;
-      const bool valid = q < KC && Q < T.ktot;
-      uint32_t gr = 0, gc = 0;
-      int e = e_cursor;
-      if (valid) {
-        while (e + 1 < e_end && entries[e + 1].kstart <= Q) ++e;
-        const Entry en = entries[e];
-        const int k = a_strips[en.a_strip].k;
-        ColInfo ci;
-        ci.a_strip = en.a_strip;
-        ci.b_strip = en.b_strip;
-        ci.kk = (int16_t)(Q - en.kstart);
-        ci.kdim = (int16_t)blk[k];
-        ci.rmask = en.rmask;
-        ci.cmask = en.cmask;
-        colinfo[q] = ci;
-        for (int p = 0; p < kBandAtoms; ++p) {
-          if ((en.rmask >> p) & 1) gr |= pg_r[p];
-          if ((en.cmask >> p) & 1) gc |= pg_c[p];
-        }
-      } else if (q < KC) {
-        ColInfo ci;
-        ci.a_strip = 0;
-        ci.b_strip = 0;
-        ci.kk = 0;
-        ci.kdim = 0;
-        ci.rmask = 0;
-        ci.cmask = 0;
-        colinfo[q] = ci;
-      }
-      // OR the group masks over each group of 4 K columns
-      gr |= __shfl_xor(gr, 1, 64);
-      gr |= __shfl_xor(gr, 2, 64);
-      gc |= __shfl_xor(gc, 1, 64);
-      gc |= __shfl_xor(gc, 2, 64);
-      if (q < KC && (q & 3) == 0) {
-        smask_r[q >> 2] = gr;
-        smask_c[q >> 2] = gc;
-      }
-      const int last = min(KC, T.ktot - Q0) - 1;
-      if (q == last) e_cursor = e;
-    }
-    __syncthreads();
-    // stage A: As[q][r]
-    {
-      const int r = tid & 63;
-      const int pos = rowpos[r];
-#pragma unroll
-      for (int p = 0; p < KC / 4; ++p) {
-        const int q = (tid >> 6) + 4 * p;
-        const ColInfo ci = colinfo[q];
-        double v = 0.0;
-        if (pos >= 0 && ((ci.rmask >> pos) & 1)) {
-          const int m = rs_r[pos + 1] - rs_r[pos];
-          const int64_t o = a_strips[ci.a_strip].off[pos];
-          v = a_data[o + (int64_t)ci.kk * m + rowin[r]];
-        }
-        As[q * kPitch + r] = v;
-      }
-    }
-    // stage B: Bs[q][c]
-    {
-      const int q = tid & 31;
-      const ColInfo ci = colinfo[q];
-#pragma unroll
-      for (int p = 0; p < kBandRows / 8; ++p) {
-        const int c = (tid >> 5) + 8 * p;
-        const int pos = colpos[c];
-        double v = 0.0;
-        if (pos >= 0 && ((ci.cmask >> pos) & 1)) {
-          const int64_t o = b_strips[ci.b_strip].off[pos];
-          v = b_data[o + (int64_t)colin[c] * ci.kdim + ci.kk];
-        }
-        Bs[q * kPitch + c] = v;
-      }
-    }
-    __syncthreads();
-    const int nsteps = (min(KC, T.ktot - Q0) + 3) / 4;
-    for (int s = 0; s < nsteps; ++s) {
-      const uint32_t mr = __builtin_amdgcn_readfirstlane(smask_r[s]);
-      const uint32_t mc = __builtin_amdgcn_readfirstlane(smask_c[s]);
-      const int krow = (4 * s + (lane >> 4)) * kPitch;
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        if (((mr >> (8 * wr + 4 * h)) & 0xFu) == 0) continue;
-        const double a_op = As[krow + 32 * wr + 16 * h + (lane & 15)];
-#pragma unroll
-        for (int g = 0; g < 8; ++g) {
-          if (((mc >> (8 * wc + g)) & 1u) == 0) continue;
-          const double b_op = Bs[krow + 32 * wc + 4 * g + (lane & 3)];
-          acc[h * 8 + g] = __builtin_amdgcn_mfma_f64_4x4x4f64(a_op, b_op, acc[h * 8 + g], 0, 0, 0);
-        }
-      }
-    }
-    __syncthreads();
+  if (nchunks > 0) LAMB_ISSUE(ra0, rb0, 0);
+  if (nchunks > 1) LAMB_ISSUE(ra1, rb1, 1);
+  if (nchunks > 0) LAMB_COMMIT(ra0, rb0, 0);
+  __syncthreads();
+  // chunk c (even): next+2 goes to set 0 (free), next+1 sits in set 1.
+  for (int c = 0; c < nchunks; c += 2) {
+    LAMB_STEP(ra0, rb0, ra1, rb1, c);
+    if (c + 1 < nchunks) LAMB_STEP(ra1, rb1, ra0, rb0, c + 1);
   }
+#undef LAMB_ISSUE
+#undef LAMB_COMMIT
+#undef LAMB_STEP
 
   // ---- epilogue: C tile to LDS ----
   double* Ct = lds;
@@ -250,49 +339,47 @@ __global__ __launch_bounds__(256) void numeric_f64_kernel(
     }
   // structural pairs (row-major over band positions) and their offsets
   const int ip = tid >> 4, jp = tid & 15;
-  const bool present = ip < bI.natoms && jp < bJ.natoms && ((T.pattern[ip] >> jp) & 1);
-  const int64_t sz = present ? (int64_t)(bI.rs[ip + 1] - bI.rs[ip]) * (bJ.rs[jp + 1] - bJ.rs[jp]) : 0;
-  // wave-inclusive scans of (present, size)
+  const bool present = ip < natI && jp < natJ && ((tp->pattern[ip] >> jp) & 1);
+  const int64_t sz =
+      present ? (int64_t)(rs_r[ip + 1] - rs_r[ip]) * (rs_c[jp + 1] - rs_c[jp]) : 0;
   int cnt = present ? 1 : 0;
-  int64_t acc_sz = sz;
+  int64_t csz = sz;
   for (int o = 1; o < 64; o <<= 1) {
     const int c2 = __shfl_up(cnt, o, 64);
-    const int64_t s2 = __shfl_up(acc_sz, o, 64);
+    const int64_t s2 = __shfl_up(csz, o, 64);
     if (lane >= o) {
       cnt += c2;
-      acc_sz += s2;
+      csz += s2;
     }
   }
-  __shared__ int64_t wave_sz[4];
   if (lane == 63) {
-    wave_tot[wave] = cnt;
-    wave_sz[wave] = acc_sz;
+    wave_cnt[wave] = cnt;
+    wave_sz[wave] = csz;
   }
   __syncthreads();
   int cbase = 0;
   int64_t sbase = 0;
   for (int w = 0; w < wave; ++w) {
-    cbase += wave_tot[w];
+    cbase += wave_cnt[w];
     sbase += wave_sz[w];
   }
-  const int npairs = wave_tot[0] + wave_tot[1] + wave_tot[2] + wave_tot[3];
+  const int npairs = wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
   if (present) {
     pair_id[cbase + cnt - 1] = tid;
-    pair_off[cbase + cnt - 1] = sbase + acc_sz - sz;
+    pair_off[cbase + cnt - 1] = sbase + csz - sz;
   }
   __syncthreads();
 
-  const int32_t atom_i0 = bI.first, atom_j0 = bJ.first;
   for (int pi = wave; pi < npairs; pi += 4) {
     const int id = pair_id[pi];
-    const int pi_ = id >> 4, pj = id & 15;
-    const int m = rs_r[pi_ + 1] - rs_r[pi_];
-    const int n = rs_c[pj + 1] - rs_c[pj];
+    const int pr = id >> 4, pc = id & 15;
+    const int m = rs_r[pr + 1] - rs_r[pr];
+    const int n = rs_c[pc + 1] - rs_c[pc];
     const int len = m * n;
-    const int64_t base = T.out_base + pair_off[pi];
+    const int64_t base = out_base + pair_off[pi];
     const double* old = nullptr;
     if (beta != 0.0 && old_row_p != nullptr) {
-      const int32_t gi = atom_i0 + pi_, gj = atom_j0 + pj;
+      const int32_t gi = firstI + pr, gj = firstJ + pc;
       int64_t lo = old_row_p[gi], hi = old_row_p[gi + 1] - 1;
       int64_t found = -1;
       while (lo <= hi) {
@@ -309,30 +396,29 @@ __global__ __launch_bounds__(256) void numeric_f64_kernel(
     }
     double part = 0.0;
     for (int e = lane; e < len; e += 64) {
-      const int c = e / m, r = e - c * m;
-      double v = alpha * Ct[(rs_r[pi_] + r) * kPitch + rs_c[pj] + c];
+      const int cc = e / m, r = e - cc * m;
+      double v = alpha * Ct[(rs_r[pr] + r) * kPitch + rs_c[pc] + cc];
       if (old) v = v + beta * old[e];
       c_data[base + e] = v;
       part = __builtin_fma(v, v, part);
     }
     const double nsq = wave_sum(part);
-    if (lane == 0 && nsq >= eps2) atomicOr(&keptw[pi_], 1u << pj);
+    if (lane == 0 && nsq >= eps2) atomicOr(&keptw[pr], 1u << pc);
   }
   __syncthreads();
-  if (tid < kBandAtoms) kept_out[(int64_t)blockIdx.x * kBandAtoms + tid] = (uint16_t)keptw[tid];
+  if (tid < kBandAtoms) kept_out[(int64_t)tile * kBandAtoms + tid] = (uint16_t)keptw[tid];
 }
 
 hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* tiles,
-                              const Entry* entries, const Strip* a_strips,
-                              const Strip* b_strips, const Band* bands, const int32_t* d_blk,
-                              const double* a_data, const double* b_data, double alpha,
+                              const Entry* entries, const Band* bands,
+                              const double* a_pack, const double* b_pack, double alpha,
                               double beta, double eps2, const int64_t* old_row_p,
                               const int32_t* old_col, const int64_t* old_off,
                               const double* old_data, double* c_data, uint16_t* kept) {
   if (n_tiles == 0) return hipSuccess;
   hipLaunchKernelGGL(numeric_f64_kernel, dim3((unsigned)n_tiles), dim3(256), 0, s, tiles,
-                     entries, a_strips, b_strips, bands, d_blk, a_data, b_data, alpha, beta,
-                     eps2, old_row_p, old_col, old_off, old_data, c_data, kept);
+                     entries, bands, a_pack, b_pack, alpha, beta, eps2, old_row_p, old_col,
+                     old_off, old_data, c_data, kept);
   return hipGetLastError();
 }
 

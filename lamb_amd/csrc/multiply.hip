@@ -175,6 +175,10 @@ extern "C" int32_t lamb_hip_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A_,
   std::vector<int64_t> a_ptr, b_ptr;
   build_a_strips(A, as, a_ptr);
   build_b_strips(B, bs, b_ptr);
+  // packed panel offsets: [kdim][64] doubles per strip
+  std::vector<int64_t> a_pk(as.size() + 1, 0), b_pk(bs.size() + 1, 0);
+  for (size_t t = 0; t < as.size(); ++t) a_pk[t + 1] = a_pk[t] + 64 * (int64_t)d->blk_int[as[t].k];
+  for (size_t t = 0; t < bs.size(); ++t) b_pk[t + 1] = b_pk[t] + 64 * (int64_t)d->blk_int[bs[t].k];
 
   // ---- symbolic: tiles and their (A strip, B strip) entries ----
   const int32_t nbands = (int32_t)d->bands.size();
@@ -207,14 +211,17 @@ extern "C" int32_t lamb_hip_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A_,
       }
     };
     auto emit = [&](int32_t sa, int32_t sb, int32_t J, int kdim, uint16_t rm, uint16_t cm) {
-      Entry e;
-      e.a_strip = sa;
-      e.b_strip = sb;
-      e.kstart = ktot[J];
-      e.rmask = rm;
-      e.cmask = cm;
-      bucket[J].push_back(e);
-      ktot[J] += kdim;
+      if (kdim > 0) {
+        Entry e{};
+        e.a_src = a_pk[sa];
+        e.b_src = b_pk[sb];
+        e.kstart = ktot[J];
+        e.kdim = (int16_t)kdim;
+        e.rmask = rm;
+        e.cmask = cm;
+        bucket[J].push_back(e);
+        ktot[J] += kdim;
+      }
       for (int p = 0; p < kBandAtoms; ++p)
         if ((rm >> p) & 1) pat[J][p] |= cm;
       kept += (int64_t)popc(rm) * popc(cm);
@@ -305,16 +312,31 @@ extern "C" int32_t lamb_hip_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A_,
   HIPCHK(ctx, ctx->d_entries.upload(s, entries.data(), sizeof(Entry) * entries.size()), "entries");
   HIPCHK(ctx, ctx->d_astrips.upload(s, as.data(), sizeof(Strip) * as.size()), "a strips");
   HIPCHK(ctx, ctx->d_bstrips.upload(s, bs.data(), sizeof(Strip) * bs.size()), "b strips");
+  HIPCHK(ctx, ctx->d_apk_off.upload(s, a_pk.data(), sizeof(int64_t) * a_pk.size()), "a pk off");
+  HIPCHK(ctx, ctx->d_bpk_off.upload(s, b_pk.data(), sizeof(int64_t) * b_pk.size()), "b pk off");
+  HIPCHK(ctx, ctx->d_apack.ensure(sizeof(double) * (size_t)std::max<int64_t>(a_pk.back(), 1)),
+         "a pack alloc");
+  HIPCHK(ctx, ctx->d_bpack.ensure(sizeof(double) * (size_t)std::max<int64_t>(b_pk.back(), 1)),
+         "b pack alloc");
+  HIPCHK(ctx, hipEventRecord(ctx->ev[2], s), "event");
+  HIPCHK(ctx, launch_pack_strips(s, (int64_t)as.size(), (const Strip*)ctx->d_astrips.p,
+                                 (const int64_t*)ctx->d_apk_off.p, (const Band*)d->d_bands.p,
+                                 (const int32_t*)d->d_blk.p, (const double*)A->d_data.p, true,
+                                 (double*)ctx->d_apack.p), "pack A");
+  HIPCHK(ctx, launch_pack_strips(s, (int64_t)bs.size(), (const Strip*)ctx->d_bstrips.p,
+                                 (const int64_t*)ctx->d_bpk_off.p, (const Band*)d->d_bands.p,
+                                 (const int32_t*)d->d_blk.p, (const double*)B->d_data.p, false,
+                                 (double*)ctx->d_bpack.p), "pack B");
+  HIPCHK(ctx, hipEventRecord(ctx->ev[3], s), "event");
   HIPCHK(ctx, ctx->d_kept.ensure(sizeof(uint16_t) * kBandAtoms * std::max(n_tiles, 1)), "kept");
   DevBuf newdata;
   HIPCHK(ctx, newdata.ensure(sizeof(double) * (size_t)std::max<int64_t>(c_size, 1)), "C pool");
 
   HIPCHK(ctx, hipEventRecord(ctx->ev[0], s), "event");
   HIPCHK(ctx, launch_numeric_f64(s, n_tiles, (const TileDesc*)ctx->d_tiles.p,
-                                 (const Entry*)ctx->d_entries.p, (const Strip*)ctx->d_astrips.p,
-                                 (const Strip*)ctx->d_bstrips.p, (const Band*)d->d_bands.p,
-                                 (const int32_t*)d->d_blk.p, (const double*)A->d_data.p,
-                                 (const double*)B->d_data.p, alpha, beta, eps2,
+                                 (const Entry*)ctx->d_entries.p, (const Band*)d->d_bands.p,
+                                 (const double*)ctx->d_apack.p, (const double*)ctx->d_bpack.p,
+                                 alpha, beta, eps2,
                                  use_old ? (const int64_t*)C->d_row_p.p : nullptr,
                                  use_old ? (const int32_t*)C->d_col.p : nullptr,
                                  use_old ? (const int64_t*)C->d_off.p : nullptr,
@@ -327,8 +349,9 @@ extern "C" int32_t lamb_hip_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A_,
     HIPCHK(ctx, hipMemcpyAsync(keptmask.data(), ctx->d_kept.p, sizeof(uint16_t) * keptmask.size(),
                                hipMemcpyDeviceToHost, s), "kept download");
   HIPCHK(ctx, hipStreamSynchronize(s), "numeric sync");
-  float ms_num = 0.f;
+  float ms_num = 0.f, ms_pack = 0.f;
   HIPCHK(ctx, hipEventElapsedTime(&ms_num, ctx->ev[0], ctx->ev[1]), "event time");
+  HIPCHK(ctx, hipEventElapsedTime(&ms_pack, ctx->ev[2], ctx->ev[3]), "event time");
 
   // ---- C index (internal CSR, rows ascending, cols ascending) ----
   std::vector<int64_t> row_p(d->n + 1, 0);
@@ -404,6 +427,7 @@ extern "C" int32_t lamb_hip_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A_,
     stats->n_tile_entries = (int64_t)entries.size();
     stats->flops = flops;
     stats->ms_numeric = ms_num;
+    stats->ms_symbolic = ms_pack;
     stats->ms_total = std::chrono::duration<double, std::milli>(
                           std::chrono::steady_clock::now() - t_start).count();
   }
