@@ -21,21 +21,20 @@ constexpr int kChunkK = 16;
 // LDS row pitch (doubles) of the staged panels; +1 breaks bank aliasing.
 constexpr int kPitch = kBandRows + 1;
 
-// One (tile, k) strip pair of the numeric kernel. a_src / b_src are the
-// element offsets of the packed A strip [kdim][64] and packed B strip
-// [kdim][64]; rmask / cmask select the atoms of the row / column band whose
-// blocks take part (bit = position of the atom inside its band).
+// One (tile, k) strip pair of the numeric kernel. a_col / b_col are the
+// first panel columns of the packed A strip [kdim][64] and packed B strip
+// [kdim][64] (element offset = col * 64); the entry covers K columns
+// [kstart, next entry's kstart) of its tile (the tile's ktot for the last).
+// rmask / cmask select the atoms of the row / column band whose blocks take
+// part (bit = position of the atom inside its band).
 struct Entry {
-  int64_t a_src;
-  int64_t b_src;
-  int32_t kstart;   // first K column of this entry within the tile
-  int16_t kdim;     // K columns of this entry (block size of atom k)
-  int16_t pad0;
+  uint32_t a_col;
+  uint32_t b_col;
+  int32_t kstart;
   uint16_t rmask;
   uint16_t cmask;
-  int32_t pad1;
 };
-static_assert(sizeof(Entry) == 32, "Entry layout");
+static_assert(sizeof(Entry) == 16, "Entry layout");
 
 struct TileDesc {
   int32_t I, J;       // row band, col band

@@ -89,36 +89,46 @@ hipError_t launch_pack_strips(hipStream_t s, int64_t n_strips, const Strip* stri
 // FP64 numeric phase.
 //
 // One workgroup (4 waves) owns one C tile = row band I x col band J, at most
-// 64 x 64 values. The tile's K dimension is the concatenation of the K atoms
-// of its entries, streamed through LDS kChunkK columns at a time:
-//   As[q][r] = packed A strip value (tile row r, K column q),
-//   Bs[q][c] = packed B strip value (K column q, tile col c).
-// Pipeline per chunk c (one barrier): issue the global loads of chunk c+1
-// into registers, wave 0 builds the column table of chunk c+2, all waves
-// run the MFMAs of chunk c, then the registers go to the other LDS buffer.
+// 64 x 64 values; wave (wr, wc) owns the 32 x 32 quadrant rows 32wr..,
+// cols 32wc.. as four 16 x 16 v_mfma_f64_16x16x4 accumulators.
 //
-// Each wave owns a 32 x 32 quadrant as 64 4x4 sub-tiles, held as 16
-// accumulators of v_mfma_f64_4x4x4_4b (4 independent 4x4x4 blocks per
-// instruction): accumulator h*8+g holds the 4 row groups of half h (blocks
-// b = 0..3) for column group g. Per 4-column K step a row-group mask and a
-// column-group mask tell which 4x4x4 products are non-zero; an instruction
-// is skipped when its column group or all four of its row groups are zero.
+// The tile's entries (strip pairs, one K atom each) are staged in LDS once;
+// then every wave streams, on its own and without barriers, the K columns
+// of the entries that touch its quadrant (an entry touches it when one of
+// its row atoms overlaps the wave's 32 rows and one of its column atoms
+// overlaps the wave's 32 columns). Four such columns form a K step. MFMA
+// operands are loaded straight from the packed [k][64] panels into a ring
+// of register slots, kPrefetch steps ahead of the MFMAs that consume them.
+// A 16 x 16 block of a step is skipped (no loads, no MFMA) when its 16 rows
+// or its 16 columns hold no atom of the step's entries; single rows /
+// columns of absent atoms are not loaded (their operand is zero).
 //
-// v_mfma_f64_4x4x4_4b lane layout (lane l): A[a][c] of block b at
-// l = a + 4b + 16c; B[c][t] of block b at l = t + 4b + 16c; D[a][t] of
-// block b at l = 16a + 4b + t.
+// v_mfma_f64_16x16x4 lane layout (lane l): A[i][k] at l = i + 16k,
+// B[k][j] at l = j + 16k, D[i][j] in lane 16*(i%4) + j, register i/4.
 // ---------------------------------------------------------------------------
-struct ColSrc {
-  int64_t a, b;  // packed source offsets of this K column (-1: zero column)
-  uint32_t rmask, cmask;
-  uint32_t groups;  // row-group mask | column-group mask << 16
-  uint32_t pad;
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+constexpr int kEntryBatch = 256;  // entries staged in LDS at a time
+constexpr int kPrefetch = 4;      // K steps in flight per wave
+
+struct LdsEntry {
+  uint32_t a_col, b_col;  // first packed panel column of the A / B strip
+  uint32_t kd;            // K columns of the entry
+  uint32_t masks;         // rmask | cmask << 16
+};
+
+// One K column of a wave's stream (wave-uniform).
+struct KCol {
+  int64_t a, b;  // panel column start (element offset), -1 = none
+  uint32_t masks;
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
 __global__ __launch_bounds__(256, 4) void numeric_f64_kernel(
     const TileDesc* __restrict__ tiles, const Entry* __restrict__ entries,
@@ -127,15 +137,11 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_kernel(
     const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
     const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
     double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
-  constexpr int KC = kChunkK;
-  constexpr int PANEL = KC * kPitch;
-  constexpr int NT = 4;  // column tables in flight (chunks c .. c+3)
-  constexpr int CT = kBandRows * kPitch;
-  __shared__ double lds[(4 * PANEL > CT) ? 4 * PANEL : CT];  // [A0 B0 A1 B1]; epilogue C tile
-  __shared__ ColSrc tab[NT][KC];
-  __shared__ uint32_t smr[NT][KC / 4], smc[NT][KC / 4];
+  // main loop: the staged entry batch; epilogue: the C tile [64][kPitch]
+  __shared__ double lds[kBandRows * kPitch];
+  static_assert(sizeof(LdsEntry) * kEntryBatch <= sizeof(double) * kBandRows * kPitch, "");
+  LdsEntry* ent = reinterpret_cast<LdsEntry*>(lds);
   __shared__ int16_t rs_r[kBandAtoms + 1], rs_c[kBandAtoms + 1];
-  __shared__ uint16_t pg_r[kBandAtoms], pg_c[kBandAtoms];
   __shared__ int32_t pair_id[kBandAtoms * kBandAtoms];
   __shared__ int64_t pair_off[kBandAtoms * kBandAtoms];
   __shared__ int32_t wave_cnt[4];
@@ -165,178 +171,181 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_kernel(
   }
   if (tid < kBandAtoms) keptw[tid] = 0;
   __syncthreads();
-  if (tid < kBandAtoms) {
-    uint32_t g = 0;
-    if (tid < natI && rs_r[tid + 1] > rs_r[tid])
-      for (int q = rs_r[tid] / 4; q <= (rs_r[tid + 1] - 1) / 4; ++q) g |= 1u << q;
-    pg_r[tid] = (uint16_t)g;
-    g = 0;
-    if (tid < natJ && rs_c[tid + 1] > rs_c[tid])
-      for (int q = rs_c[tid] / 4; q <= (rs_c[tid + 1] - 1) / 4; ++q) g |= 1u << q;
-    pg_c[tid] = (uint16_t)g;
+
+  // atoms overlapping this wave's 16-row halves / 16-column quads
+  uint32_t hm_r[2] = {0u, 0u}, hm_c[2] = {0u, 0u};
+  for (int a = 0; a < natI; ++a) {
+    const int r0 = rs_r[a], r1 = rs_r[a + 1];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int lo = 32 * wr + 16 * h;
+      if (r1 > r0 && r0 < lo + 16 && r1 > lo) hm_r[h] |= 1u << a;
+    }
   }
-  // band position of this thread's staging row (A) and column (B)
-  int rpos = -1, cpos = -1;
+  for (int a = 0; a < natJ; ++a) {
+    const int c0 = rs_c[a], c1 = rs_c[a + 1];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int lo = 32 * wc + 16 * q;
+      if (c1 > c0 && c0 < lo + 16 && c1 > lo) hm_c[q] |= 1u << a;
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    hm_r[h] = uni(hm_r[h]);
+    hm_c[h] = uni(hm_c[h]);
+  }
+  const uint32_t wrows = hm_r[0] | hm_r[1], wcols = hm_c[0] | hm_c[1];
+  // band position of this lane's operand rows (A) and columns (B)
+  int rpos[2] = {-1, -1}, cpos[2] = {-1, -1};
   for (int a = 0; a < natI; ++a)
-    if (lane >= rs_r[a] && lane < rs_r[a + 1]) rpos = a;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int r = 32 * wr + 16 * h + (lane & 15);
+      if (r >= rs_r[a] && r < rs_r[a + 1]) rpos[h] = a;
+    }
   for (int a = 0; a < natJ; ++a)
-    if (lane >= rs_c[a] && lane < rs_c[a + 1]) cpos = a;
-  __syncthreads();
-
-  const int nchunks = (ktot + KC - 1) / KC;
-  const int e_end = e0 + ne;
-
-  // Column table of chunk ch into tab[buf] / smr[buf] / smc[buf], built by
-  // wave 0. The entry cursor is a wave-uniform register of wave 0; lanes
-  // exchange data through LDS only across an explicit wavefront fence and
-  // wave barrier.
-  int cursor = e0;
-  auto build_table = [&](int ch, int buf) {
-    const int Q0 = ch * KC;
-    if (lane < KC && Q0 + lane >= ktot) {
-      ColSrc z;
-      z.a = -1;
-      z.b = -1;
-      z.rmask = 0;
-      z.cmask = 0;
-      z.groups = 0;
-      z.pad = 0;
-      tab[buf][lane] = z;
-    }
-    const int e = cursor + lane;
-    const bool valid = lane <= KC && e < e_end;
-    int ks = 0, kd = 0;
-    int64_t asrc = 0, bsrc = 0;
-    uint32_t rm = 0, cm = 0;
-    if (valid) {
-      const Entry* en = entries + e;
-      ks = en->kstart;
-      kd = en->kdim;
-      asrc = en->a_src;
-      bsrc = en->b_src;
-      rm = en->rmask;
-      cm = en->cmask;
-    }
-    const int lo = max(ks, Q0), hi = min(ks + kd, Q0 + KC);
-    if (valid && lo < hi) {
-      uint32_t gr = 0, gc = 0;
-      for (int p = 0; p < kBandAtoms; ++p) {
-        if ((rm >> p) & 1u) gr |= pg_r[p];
-        if ((cm >> p) & 1u) gc |= pg_c[p];
-      }
-      for (int Q = lo; Q < hi; ++Q) {
-        ColSrc cs;
-        cs.a = asrc + (int64_t)(Q - ks) * 64;
-        cs.b = bsrc + (int64_t)(Q - ks) * 64;
-        cs.rmask = rm;
-        cs.cmask = cm;
-        cs.groups = gr | (gc << 16);
-        cs.pad = 0;
-        tab[buf][Q - Q0] = cs;
-      }
-    }
-    const uint64_t done = __ballot(valid && ks + kd <= Q0 + KC);
-    cursor += (int)__popcll(done);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (lane < KC / 4) {
-      uint32_t g = 0;
 #pragma unroll
-      for (int t = 0; t < 4; ++t) g |= tab[buf][4 * lane + t].groups;
-      smr[buf][lane] = g & 0xFFFFu;
-      smc[buf][lane] = g >> 16;
+    for (int q = 0; q < 2; ++q) {
+      const int c = 32 * wc + 16 * q + (lane & 15);
+      if (c >= rs_c[a] && c < rs_c[a + 1]) cpos[q] = a;
     }
-  };
+  const int kc = lane >> 4;  // K column of this lane within a step
 
-  if (wave == 0)
-    for (int ch = 0; ch < min(nchunks, NT - 1); ++ch) build_table(ch, ch);
-  __syncthreads();
-
-  // Two register sets of staged loads: chunk c+1 and c+2 are in flight while
-  // chunk c runs on the MFMA pipes. Loads are predicated on the entry's atom
-  // masks, so rows / columns of absent atoms cost no memory traffic.
-  constexpr int PER = KC / 4;  // K columns per wave per chunk
-  double ra0[PER], rb0[PER], ra1[PER], rb1[PER];
-#define LAMB_ISSUE(RA, RB, ch)                                             \
-  {                                                                        \
-    const int t_ = (ch) % NT;                                              \
-    _Pragma("unroll") for (int p = 0; p < PER; ++p) {                      \
-      const int q = wave + 4 * p;                                          \
-      const int64_t sa = tab[t_][q].a, sb = tab[t_][q].b;                  \
-      const uint32_t rmk = tab[t_][q].rmask, cmk = tab[t_][q].cmask;       \
-      RA[p] = (sa >= 0 && rpos >= 0 && ((rmk >> rpos) & 1u)) ? a_pack[sa + lane] : 0.0; \
-      RB[p] = (sb >= 0 && cpos >= 0 && ((cmk >> cpos) & 1u)) ? b_pack[sb + lane] : 0.0; \
-    }                                                                      \
-  }
-#define LAMB_COMMIT(RA, RB, ch)                                            \
-  {                                                                        \
-    double* As_ = lds + ((ch) & 1) * 2 * PANEL;                            \
-    double* Bs_ = As_ + PANEL;                                             \
-    _Pragma("unroll") for (int p = 0; p < PER; ++p) {                      \
-      const int q = wave + 4 * p;                                          \
-      As_[q * kPitch + lane] = RA[p];                                      \
-      Bs_[q * kPitch + lane] = RB[p];                                      \
-    }                                                                      \
-  }
-#define LAMB_STEP(RA_NEXT2, RB_NEXT2, RA_NEXT1, RB_NEXT1, c)               \
-  {                                                                        \
-    if ((c) + 2 < nchunks) LAMB_ISSUE(RA_NEXT2, RB_NEXT2, (c) + 2);        \
-    if (wave == 0 && (c) + NT - 1 < nchunks) build_table((c) + NT - 1, ((c) + NT - 1) % NT); \
-    {                                                                      \
-      const double* As = lds + ((c) & 1) * 2 * PANEL;                      \
-      const double* Bs = As + PANEL;                                       \
-      const int t3 = (c) % NT;                                             \
-      const int nsteps = (min(KC, ktot - (c) * KC) + 3) >> 2;              \
-      for (int s = 0; s < nsteps; ++s) {                                   \
-        const uint32_t mr = __builtin_amdgcn_readfirstlane(smr[t3][s]) >> (8 * wr); \
-        const uint32_t mc = __builtin_amdgcn_readfirstlane(smc[t3][s]) >> (8 * wc); \
-        if ((mr & 0xFFu) == 0 || (mc & 0xFFu) == 0) continue;             \
-        const int krow = (4 * s + (lane >> 4)) * kPitch;                   \
-        double bop[8];                                                     \
-        _Pragma("unroll") for (int g = 0; g < 8; ++g)                      \
-          bop[g] = ((mc >> g) & 1u) ? Bs[krow + 32 * wc + 4 * g + (lane & 3)] : 0.0; \
-        _Pragma("unroll") for (int h = 0; h < 2; ++h) {                    \
-          if ((mr >> (4 * h)) & 0xFu) {                                    \
-            const double aop = As[krow + 32 * wr + 16 * h + (lane & 15)];  \
-            _Pragma("unroll") for (int g = 0; g < 8; ++g)                  \
-              if ((mc >> g) & 1u)                                          \
-                acc[h * 8 + g] = __builtin_amdgcn_mfma_f64_4x4x4f64(aop, bop[g], acc[h * 8 + g], 0, 0, 0); \
-          }                                                                \
-        }                                                                  \
-      }                                                                    \
-    }                                                                      \
-    if ((c) + 1 < nchunks) LAMB_COMMIT(RA_NEXT1, RB_NEXT1, (c) + 1);       \
-    __syncthreads();                                                       \
-  }
-
-  double acc[16];
+  dbl4 acc[4];
 #pragma unroll
-  for (int x = 0; x < 16; ++x) acc[x] = 0.0;
+  for (int x = 0; x < 4; ++x) acc[x] = dbl4{0.0, 0.0, 0.0, 0.0};
 
-  if (nchunks > 0) LAMB_ISSUE(ra0, rb0, 0);
-  if (nchunks > 1) LAMB_ISSUE(ra1, rb1, 1);
-  if (nchunks > 0) LAMB_COMMIT(ra0, rb0, 0);
-  __syncthreads();
-  // chunk c (even): next+2 goes to set 0 (free), next+1 sits in set 1.
-  for (int c = 0; c < nchunks; c += 2) {
-    LAMB_STEP(ra0, rb0, ra1, rb1, c);
-    if (c + 1 < nchunks) LAMB_STEP(ra1, rb1, ra0, rb0, c + 1);
+  for (int b0 = 0; b0 < ne; b0 += kEntryBatch) {
+    const int nbatch = min(kEntryBatch, ne - b0);
+    if (b0 > 0) __syncthreads();  // every wave is done with the previous batch
+    if (tid < nbatch) {
+      const int e = e0 + b0 + tid;
+      const Entry en = entries[e];
+      const int ks_next = (b0 + tid + 1 < ne) ? entries[e + 1].kstart : ktot;
+      LdsEntry le;
+      le.a_col = en.a_col;
+      le.b_col = en.b_col;
+      le.kd = (uint32_t)(ks_next - en.kstart);
+      le.masks = (uint32_t)en.rmask | ((uint32_t)en.cmask << 16);
+      ent[tid] = le;
+    }
+    __syncthreads();
+
+    // ---- per-wave stream over this batch ----
+    int ei = 0;        // next entry of the batch
+    uint32_t kk = 0, kd = 0, acol = 0, bcol = 0, msk = 0;
+    // next relevant column of the stream (wave-uniform), or a = -1
+    auto next_col = [&]() -> KCol {
+      while (kk >= kd && ei < nbatch) {
+        const LdsEntry le = ent[ei];
+        ++ei;
+        acol = uni(le.a_col);
+        bcol = uni(le.b_col);
+        kd = uni(le.kd);
+        msk = uni(le.masks);
+        kk = ((msk & 0xFFFFu) & wrows) && ((msk >> 16) & wcols) ? 0u : kd;
+      }
+      KCol col;
+      if (kk < kd) {
+        col.a = (int64_t)(acol + kk) * 64;
+        col.b = (int64_t)(bcol + kk) * 64;
+        col.masks = msk;
+        ++kk;
+      } else {
+        col.a = -1;
+        col.b = -1;
+        col.masks = 0;
+      }
+      return col;
+    };
+
+    // ring of kPrefetch steps: operands + which 16x16 blocks run
+    double ra[kPrefetch][2], rb[kPrefetch][2];
+    uint32_t blkmask[kPrefetch];  // bit 2h+q: block (h, q) runs
+    bool more = true;
+    // Assemble the next step and issue its loads into ring slot S.
+#define LAMB_FETCH(S)                                                           \
+    {                                                                           \
+      KCol c0 = next_col(), c1 = next_col(), c2 = next_col(), c3 = next_col(); \
+      more = c3.a >= 0;                                                         \
+      uint32_t hb = 0, qb = 0;                                                  \
+      const uint32_t m0 = c0.masks, m1 = c1.masks, m2 = c2.masks, m3 = c3.masks; \
+      const uint32_t mall = m0 | m1 | m2 | m3;                                  \
+      _Pragma("unroll") for (int h = 0; h < 2; ++h)                             \
+        if (mall & hm_r[h]) hb |= 1u << h;                                      \
+      _Pragma("unroll") for (int q = 0; q < 2; ++q)                             \
+        if ((mall >> 16) & hm_c[q]) qb |= 1u << q;                              \
+      uint32_t bm = 0;                                                          \
+      _Pragma("unroll") for (int h = 0; h < 2; ++h)                             \
+        _Pragma("unroll") for (int q = 0; q < 2; ++q)                           \
+          if (((hb >> h) & 1u) && ((qb >> q) & 1u)) bm |= 1u << (2 * h + q);    \
+      blkmask[S] = bm;                                                          \
+      const int64_t la = kc == 0 ? c0.a : kc == 1 ? c1.a : kc == 2 ? c2.a : c3.a; \
+      const int64_t lb = kc == 0 ? c0.b : kc == 1 ? c1.b : kc == 2 ? c2.b : c3.b; \
+      const uint32_t lm = kc == 0 ? m0 : kc == 1 ? m1 : kc == 2 ? m2 : m3;     \
+      _Pragma("unroll") for (int h = 0; h < 2; ++h) {                           \
+        double v = 0.0;                                                         \
+        if ((bm & (3u << (2 * h))) && la >= 0 && rpos[h] >= 0 && ((lm >> rpos[h]) & 1u)) \
+          v = a_pack[la + 32 * wr + 16 * h + (lane & 15)];                      \
+        ra[S][h] = v;                                                           \
+      }                                                                         \
+      _Pragma("unroll") for (int q = 0; q < 2; ++q) {                           \
+        double v = 0.0;                                                         \
+        if ((bm & (5u << q)) && lb >= 0 && cpos[q] >= 0 && ((lm >> (16 + cpos[q])) & 1u)) \
+          v = b_pack[lb + 32 * wc + 16 * q + (lane & 15)];                      \
+        rb[S][q] = v;                                                           \
+      }                                                                         \
+    }
+#define LAMB_MFMA(S)                                                            \
+    {                                                                           \
+      const uint32_t bm = blkmask[S];                                           \
+      _Pragma("unroll") for (int h = 0; h < 2; ++h)                             \
+        _Pragma("unroll") for (int q = 0; q < 2; ++q)                           \
+          if ((bm >> (2 * h + q)) & 1u)                                         \
+            acc[2 * h + q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[S][h], rb[S][q], acc[2 * h + q], 0, 0, 0); \
+    }
+    // prologue: fill the ring
+    _Pragma("unroll") for (int S = 0; S < kPrefetch; ++S) {
+      if (more) {
+        LAMB_FETCH(S);
+      } else {
+        blkmask[S] = 0;
+      }
+    }
+    // steady state: consume slot S, refill it kPrefetch steps ahead
+    bool live = true;
+    while (live) {
+      live = false;
+      _Pragma("unroll") for (int S = 0; S < kPrefetch; ++S) {
+        if (blkmask[S]) live = true;
+        LAMB_MFMA(S);
+        if (more) {
+          LAMB_FETCH(S);
+          live = true;
+        } else {
+          blkmask[S] = 0;
+        }
+      }
+    }
+#undef LAMB_FETCH
+#undef LAMB_MFMA
   }
-#undef LAMB_ISSUE
-#undef LAMB_COMMIT
-#undef LAMB_STEP
+  __syncthreads();
 
   // ---- epilogue: C tile to LDS ----
+  // v_mfma_f64_16x16x4 result: D[i][j] in lane 16*(i%4) + j, register i/4.
   double* Ct = lds;
 #pragma unroll
   for (int h = 0; h < 2; ++h)
 #pragma unroll
-    for (int g = 0; g < 8; ++g) {
-      const int row = 32 * wr + 16 * h + 4 * ((lane >> 2) & 3) + (lane >> 4);
-      const int col = 32 * wc + 4 * g + (lane & 3);
-      Ct[row * kPitch + col] = acc[h * 8 + g];
-    }
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 32 * wr + 16 * h + 4 * r + (lane >> 4);
+        const int col = 32 * wc + 16 * q + (lane & 15);
+        Ct[row * kPitch + col] = acc[2 * h + q][r];
+      }
   // structural pairs (row-major over band positions) and their offsets
   const int ip = tid >> 4, jp = tid & 15;
   const bool present = ip < natI && jp < natJ && ((tp->pattern[ip] >> jp) & 1);

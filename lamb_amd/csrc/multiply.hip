@@ -213,10 +213,9 @@ extern "C" int32_t lamb_hip_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A_,
     auto emit = [&](int32_t sa, int32_t sb, int32_t J, int kdim, uint16_t rm, uint16_t cm) {
       if (kdim > 0) {
         Entry e{};
-        e.a_src = a_pk[sa];
-        e.b_src = b_pk[sb];
+        e.a_col = (uint32_t)(a_pk[sa] / 64);
+        e.b_col = (uint32_t)(b_pk[sb] / 64);
         e.kstart = ktot[J];
-        e.kdim = (int16_t)kdim;
         e.rmask = rm;
         e.cmask = cm;
         bucket[J].push_back(e);
@@ -301,6 +300,7 @@ extern "C" int32_t lamb_hip_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A_,
     }
   }
   if (entries.size() > (size_t)INT32_MAX || as.size() > (size_t)INT32_MAX ||
+      a_pk.back() / 64 > (int64_t)UINT32_MAX || b_pk.back() / 64 > (int64_t)UINT32_MAX ||
       bs.size() > (size_t)INT32_MAX || tiles.size() > (size_t)INT32_MAX)
     return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_OVERFLOW_ERROR,
                      "multiply: symbolic sizes exceed 32-bit indices");
