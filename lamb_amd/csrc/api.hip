@@ -99,6 +99,10 @@ int32_t lamb_hip_finalize(lamb_hip_ctx* ctx) {
   if (!ctx) return LAMB_HIP_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  release_sym_state(ctx);
+  ctx->d_kept.release();
+  ctx->d_apack.release();
+  ctx->d_bpack.release();
   for (auto& ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -202,6 +206,10 @@ int32_t lamb_hip_dist_create(lamb_hip_ctx* ctx, int64_t n_atoms, const int8_t* b
   HIPCHK(ctx, d->d_blk.upload(ctx->stream, d->blk_int.data(), sizeof(int32_t) * d->n), "dist blk");
   HIPCHK(ctx, d->d_bands.upload(ctx->stream, d->bands.data(), sizeof(Band) * d->bands.size()),
          "dist bands");
+  HIPCHK(ctx, d->d_band_of.upload(ctx->stream, d->band_of.data(), sizeof(int32_t) * d->n),
+         "dist band_of");
+  HIPCHK(ctx, d->d_pos_in_band.upload(ctx->stream, d->pos_in_band.data(), sizeof(int32_t) * d->n),
+         "dist pos_in_band");
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "dist sync");
   *out = d.release();
   return LAMB_HIP_OK;
@@ -212,6 +220,8 @@ int32_t lamb_hip_dist_destroy(lamb_hip_dist* d) {
   (void)hipSetDevice(d->ctx->device);
   d->d_blk.release();
   d->d_bands.release();
+  d->d_band_of.release();
+  d->d_pos_in_band.release();
   delete d;
   return LAMB_HIP_OK;
 }
@@ -260,6 +270,20 @@ int32_t lamb_hip_matrix_copy(const lamb_hip_mat* src, lamb_hip_mat* dst) {
                      "matrix_copy: matrices have different distributions");
   if (src == dst) return LAMB_HIP_OK;
   (void)hipSetDevice(ctx->device);
+  hipStream_t s = ctx->stream;
+  auto d2d = [&](DevBuf& to, const DevBuf& from, size_t bytes) -> hipError_t {
+    hipError_t e = to.ensure(std::max<size_t>(bytes, 8));
+    if (e != hipSuccess || bytes == 0) return e;
+    return hipMemcpyAsync(to.p, from.p, bytes, hipMemcpyDeviceToDevice, s);
+  };
+  const size_t nb = (size_t)src->nblk;
+  HIPCHK(ctx, d2d(dst->d_data, src->d_data, sizeof(double) * (size_t)src->ndata), "copy data");
+  HIPCHK(ctx, d2d(dst->d_row_p, src->d_row_p, sizeof(int64_t) * (size_t)(src->dist->n + 1)),
+         "copy row_p");
+  HIPCHK(ctx, d2d(dst->d_col, src->d_col, sizeof(int32_t) * nb), "copy col");
+  HIPCHK(ctx, d2d(dst->d_off, src->d_off, sizeof(int64_t) * nb), "copy off");
+  HIPCHK(ctx, d2d(dst->d_rows, src->d_rows, sizeof(int32_t) * nb), "copy rows");
+  dst->host_index_valid = src->host_index_valid;
   dst->row_p = src->row_p;
   dst->col = src->col;
   dst->off = src->off;
@@ -270,15 +294,35 @@ int32_t lamb_hip_matrix_copy(const lamb_hip_mat* src, lamb_hip_mat* dst) {
   dst->invalidate_derived();
   dst->staged_sectors.clear();
   dst->staged_lists.clear();
-  const size_t nb = (size_t)src->nblk;
-  HIPCHK(ctx, dst->d_data.ensure(sizeof(double) * (size_t)src->ndata), "copy data alloc");
-  if (src->ndata)
-    HIPCHK(ctx, hipMemcpyAsync(dst->d_data.p, src->d_data.p, sizeof(double) * src->ndata,
-                               hipMemcpyDeviceToDevice, ctx->stream), "copy data");
-  HIPCHK(ctx, dst->upload_index(ctx->stream), "copy index");
-  (void)nb;
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "copy sync");
+  HIPCHK(ctx, hipStreamSynchronize(s), "copy sync");
   return LAMB_HIP_OK;
+}
+
+hipError_t lamb_hip_mat::download_index(hipStream_t s) {
+  if (host_index_valid) return hipSuccess;
+  const size_t nb = (size_t)nblk;
+  row_p.resize((size_t)dist->n + 1);
+  col.resize(nb);
+  off.resize(nb);
+  rows.resize(nb);
+  hipError_t e;
+  if ((e = hipMemcpyAsync(row_p.data(), d_row_p.p, sizeof(int64_t) * row_p.size(),
+                          hipMemcpyDeviceToHost, s)) != hipSuccess)
+    return e;
+  if (nb) {
+    if ((e = hipMemcpyAsync(col.data(), d_col.p, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, s)) !=
+        hipSuccess)
+      return e;
+    if ((e = hipMemcpyAsync(off.data(), d_off.p, sizeof(int64_t) * nb, hipMemcpyDeviceToHost, s)) !=
+        hipSuccess)
+      return e;
+    if ((e = hipMemcpyAsync(rows.data(), d_rows.p, sizeof(int32_t) * nb, hipMemcpyDeviceToHost,
+                            s)) != hipSuccess)
+      return e;
+  }
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+  host_index_valid = true;
+  return hipSuccess;
 }
 
 hipError_t lamb_hip_mat::upload_index(hipStream_t s) {
@@ -424,6 +468,7 @@ int32_t lamb_hip_matrix_commit(lamb_hip_mat* m) {
   for (int32_t p = 0; p < d->n; ++p) m->row_p[p + 1] += m->row_p[p];
   m->nblk = nb;
   m->ndata = total;
+  m->host_index_valid = true;
   m->invalidate_derived();
   HIPCHK(ctx, m->d_data.ensure(sizeof(double) * (size_t)std::max<int64_t>(total, 1)), "pool alloc");
   size_t si = 0;
@@ -495,6 +540,8 @@ int32_t lamb_hip_sector_query(lamb_hip_mat* m, int32_t sr, int32_t sc, int64_t* 
   if (sr < 1 || sr > d->n_subsets || sc < 1 || sc > d->n_subsets)
     return set_error(d->ctx, LAMB_HIP_FAILURE, LAMB_HIP_WRONG_ARGS,
                      "sector_query: subsets (%d,%d) outside 1..%d", sr, sc, d->n_subsets);
+  (void)hipSetDevice(d->ctx->device);
+  HIPCHK(d->ctx, m->download_index(d->ctx->stream), "index download");
   std::vector<std::pair<int32_t, int64_t>> rb;
   std::vector<int64_t> rs;
   sector_blocks(m, sr, sc, rb, rs);
@@ -514,6 +561,7 @@ int32_t lamb_hip_sector_download(lamb_hip_mat* m, int32_t sr, int32_t sc, int64_
     return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_WRONG_ARGS,
                      "sector_download: subsets (%d,%d) outside 1..%d", sr, sc, d->n_subsets);
   (void)hipSetDevice(ctx->device);
+  HIPCHK(ctx, m->download_index(ctx->stream), "index download");
   int32_t rc = ensure_host_data(m);
   if (rc) return rc;
   std::vector<std::pair<int32_t, int64_t>> rb;

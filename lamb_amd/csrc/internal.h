@@ -38,6 +38,9 @@ struct StagedList {
 int32_t set_error(lamb_hip_ctx* ctx, int32_t level, int32_t type, const char* fmt, ...);
 int32_t hip_error(lamb_hip_ctx* ctx, hipError_t e, const char* what);
 
+// Releases the device scratch of the symbolic phase held by a context.
+void release_sym_state(lamb_hip_ctx* ctx);
+
 // Recursive coordinate bisection order of n atoms (coords: xyz per atom).
 void spatial_order(int32_t n, const float* coords, std::vector<int32_t>& order);
 
@@ -50,9 +53,9 @@ struct lamb_hip_ctx {
   int32_t level = 0;
   int32_t type = 0;
   std::string msg;
-  // scratch reused across multiplies
-  lamb::DevBuf d_tiles, d_entries, d_astrips, d_bstrips, d_kept;
-  lamb::DevBuf d_apk_off, d_bpk_off, d_apack, d_bpack;
+  // scratch reused across multiplies (symbolic.h)
+  lamb::DevBuf d_kept, d_apack, d_bpack;
+  void* sym = nullptr;  // lamb::SymState*
 };
 
 struct lamb_hip_dist {
@@ -72,11 +75,14 @@ struct lamb_hip_dist {
   std::vector<int32_t> pos_in_band;  // by internal atom
   lamb::DevBuf d_blk;    // int32 by internal atom
   lamb::DevBuf d_bands;  // Band[]
+  lamb::DevBuf d_band_of, d_pos_in_band;  // int32 by internal atom
 };
 
 struct lamb_hip_mat {
   lamb_hip_dist* dist = nullptr;
-  // internal block-CSR (rows/cols are internal atom indices, cols ascending)
+  // internal block-CSR (rows/cols are internal atom indices, cols ascending);
+  // the device arrays are authoritative, the host vectors a cache
+  bool host_index_valid = true;
   std::vector<int64_t> row_p;
   std::vector<int32_t> col;
   std::vector<int64_t> off;
@@ -93,6 +99,7 @@ struct lamb_hip_mat {
   std::vector<lamb::StagedList> staged_lists;
 
   hipError_t upload_index(hipStream_t s);
+  hipError_t download_index(hipStream_t s);
   void release_device();
   void invalidate_derived();
 };

@@ -1,0 +1,81 @@
+// Device symbolic phase: strips, tiles / entries, C index.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "common.h"
+#include "internal.h"
+
+namespace lamb {
+
+// device arrays of a distribution
+struct DistView {
+  int32_t n = 0;       // atoms
+  int32_t nbands = 0;  // bands
+  const int32_t* blk = nullptr;          // by internal atom
+  const Band* bands = nullptr;
+  const int32_t* band_of = nullptr;      // by internal atom
+  const int32_t* pos_in_band = nullptr;  // by internal atom
+};
+
+// device arrays of a matrix (internal block-CSR)
+struct MatView {
+  int64_t nblk = 0;
+  const int64_t* row_p = nullptr;
+  const int32_t* col = nullptr;
+  const int64_t* off = nullptr;
+  const int32_t* rows = nullptr;
+  const double* nsq = nullptr;
+};
+
+struct StripSet {
+  int64_t n = 0;         // strips
+  int64_t pk_total = 0;  // packed panel elements
+  DevBuf strips;         // Strip[n]
+  DevBuf ptr;            // int64: per band (A) or per row atom (B), +1
+  DevBuf pk;             // int64[n+1]: packed panel offsets
+  void release() {
+    strips.release();
+    ptr.release();
+    pk.release();
+  }
+};
+
+struct SymScratch {
+  DevBuf merged, flag, excl, scan, gslot;
+  DevBuf cnt, ksum, pat, stats, band4, band4s;
+  void release() {
+    for (DevBuf* b : {&merged, &flag, &excl, &scan, &gslot, &cnt, &ksum, &pat, &stats, &band4,
+                      &band4s})
+      b->release();
+  }
+};
+
+struct SymResult {
+  int64_t n_tiles = 0, n_entries = 0, c_size = 0, n_struct = 0;
+  int64_t kept = 0, dropped = 0;
+  double flops = 0.0;
+  DevBuf tiles, entries, tile_base;
+  void release() {
+    tiles.release();
+    entries.release();
+    tile_base.release();
+  }
+};
+
+struct CIndex {
+  int64_t nblk = 0, nelems = 0;
+  DevBuf row_p, col, off, rows;
+};
+
+hipError_t build_strips(hipStream_t s, SymScratch& W, const MatView& M, const DistView& D,
+                        bool a_side, StripSet& out);
+hipError_t build_tiles(hipStream_t s, SymScratch& W, const DistView& D, const StripSet& A,
+                       const StripSet& B, const MatView* Cold, double eps2, SymResult& r);
+hipError_t build_c_index(hipStream_t s, SymScratch& W, const DistView& D, const SymResult& r,
+                         const uint16_t* kept, CIndex& c);
+
+}  // namespace lamb

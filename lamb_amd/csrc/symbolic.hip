@@ -1,0 +1,676 @@
+// Symbolic phase on the device: block norms -> strips -> packed panels ->
+// tiles and their (A strip, B strip) entries, and the C index after the
+// numeric kernel. Every step is deterministic: entries of a tile are in
+// ascending K atom order and C blocks in ascending column order.
+#include "common.h"
+#include "device_util.h"
+#include "symbolic.h"
+
+namespace lamb {
+
+__global__ void fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+static hipError_t fill32(hipStream_t s, int32_t* p, int64_t n, int32_t v) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// A strips. The blocks of a band's rows, merged by column atom k (ties by
+// row position), are the band's strips in order. Each block finds its merged
+// position with one binary search per row of its band.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int64_t lower_bound_i32(const int32_t* a, int64_t lo, int64_t hi,
+                                                   int32_t key) {
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void a_merge_rank(int64_t nblk, const int32_t* __restrict__ rows,
+                             const int32_t* __restrict__ col, const int64_t* __restrict__ row_p,
+                             const int32_t* __restrict__ band_of,
+                             const int32_t* __restrict__ pos_in_band,
+                             const Band* __restrict__ bands, int64_t* __restrict__ merged) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblk) return;
+  const int32_t i = rows[b];
+  const int32_t k = col[b];
+  const int32_t I = band_of[i];
+  const int p = pos_in_band[i];
+  const int32_t first = bands[I].first;
+  const int nat = bands[I].natoms;
+  int64_t pos = row_p[first];
+  for (int q = 0; q < nat; ++q) {
+    const int64_t lo = row_p[first + q], hi = row_p[first + q + 1];
+    if (q == p) {
+      pos += b - lo;
+    } else {
+      // rows before p sort ahead on ties
+      int64_t c = lower_bound_i32(col, lo, hi, q < p ? k + 1 : k);
+      pos += c - lo;
+    }
+  }
+  merged[pos] = b;
+}
+
+// strip-start flags over a sequence of blocks: A (merged order) or B (CSR
+// order). A strip starts where the band changes or the K atom changes.
+__global__ void strip_flags(int64_t nblk, int a_side, const int64_t* __restrict__ merged,
+                            const int32_t* __restrict__ rows, const int32_t* __restrict__ col,
+                            const int32_t* __restrict__ band_of, int64_t* __restrict__ flag) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nblk) return;
+  int64_t f;
+  if (a_side) {
+    const int64_t b = merged[t];
+    if (t == 0) {
+      f = 1;
+    } else {
+      const int64_t pb = merged[t - 1];
+      f = (band_of[rows[b]] != band_of[rows[pb]] || col[b] != col[pb]) ? 1 : 0;
+    }
+  } else {
+    f = (t == 0 || rows[t] != rows[t - 1] || band_of[col[t]] != band_of[col[t - 1]]) ? 1 : 0;
+  }
+  flag[t] = f;
+}
+
+__global__ void strip_init(int64_t n, Strip* __restrict__ s) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  s[i].mask = 0;
+  s[i].pad0 = 0;
+  s[i].pad1 = 0;
+  s[i].min_nsq = __builtin_inf();
+}
+
+// scan[t] = strip index + 1 of the block at sequence slot t (inclusive count)
+__global__ void strip_fill(int64_t nblk, int a_side, const int64_t* __restrict__ merged,
+                           const int64_t* __restrict__ excl, const int64_t* __restrict__ flag,
+                           const int32_t* __restrict__ rows, const int32_t* __restrict__ col,
+                           const int64_t* __restrict__ off, const double* __restrict__ nsq,
+                           const int32_t* __restrict__ band_of,
+                           const int32_t* __restrict__ pos_in_band, Strip* __restrict__ strips) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nblk) return;
+  const int64_t b = a_side ? merged[t] : t;
+  const int64_t s = excl[t] + flag[t] - 1;
+  const int32_t i = rows[b], j = col[b];
+  // A strip (I, k): position of row atom i; B strip (k, J): position of col atom j
+  const int p = a_side ? pos_in_band[i] : pos_in_band[j];
+  Strip* S = strips + s;
+  if (flag[t]) {
+    S->k = a_side ? j : i;
+    S->band = a_side ? band_of[i] : band_of[j];
+  }
+  S->off[p] = off[b];
+  S->nsq[p] = nsq[b];
+  atomicOr(reinterpret_cast<uint32_t*>(&S->mask), 1u << p);
+  atomicMin(reinterpret_cast<unsigned long long*>(&S->min_nsq),
+            (unsigned long long)__double_as_longlong(nsq[b]));
+}
+
+// per-strip panel size (K columns x 64)
+__global__ void strip_panel_size(int64_t n, const Strip* __restrict__ s,
+                                 const int32_t* __restrict__ blk, int64_t* __restrict__ sz) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) sz[i] = 64 * (int64_t)blk[s[i].k];
+}
+
+// ptr[g] = first strip of group g: strip index at the group's first slot
+__global__ void group_ptr(int64_t ngroups, const int64_t* __restrict__ slot_of_group,
+                          const int64_t* __restrict__ excl, int64_t nslots, int64_t nstrips,
+                          int64_t* __restrict__ ptr) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g > ngroups) return;
+  if (g == ngroups) {
+    ptr[g] = nstrips;
+    return;
+  }
+  const int64_t t = slot_of_group[g];
+  ptr[g] = t < nslots ? excl[t] : nstrips;
+}
+
+// slot of band I's first block in the merged A order = row_p[first atom]
+__global__ void band_first_slot(int32_t nbands, const Band* __restrict__ bands,
+                                const int64_t* __restrict__ row_p, int64_t* __restrict__ out) {
+  const int32_t I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I < nbands) out[I] = row_p[bands[I].first];
+}
+
+// ---------------------------------------------------------------------------
+// Tiles and entries. One wave per (row band I, segment g) walks its share of
+// band I's A strips in K order; the lanes take the B strips of that K atom
+// (distinct column bands J). Counters per (I, g, J) live in dense scratch.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int rows_of_mask(const int16_t* rs, uint32_t m) {
+  int r = 0;
+  for (int p = 0; p < kBandAtoms; ++p)
+    if ((m >> p) & 1u) r += rs[p + 1] - rs[p];
+  return r;
+}
+
+// Calls emit(rm, cm) for every entry of strip pair (SA, SB) under the
+// prefilter nsq(A_ik) * nsq(B_kj) >= eps2; returns the dropped pair count.
+template <typename F>
+__device__ __forceinline__ int pair_entries(const Strip* SA, const Strip* SB, double eps2,
+                                            F&& emit) {
+  const uint32_t am = SA->mask, bm = SB->mask;
+  if (SA->min_nsq * SB->min_nsq >= eps2) {
+    emit(am, bm);
+    return 0;
+  }
+  int dropped = 0;
+  for (int q = 0; q < kBandAtoms; ++q) {
+    if (!((bm >> q) & 1u)) continue;
+    uint32_t r = 0;
+    for (int p = 0; p < kBandAtoms; ++p)
+      if (((am >> p) & 1u) && SA->nsq[p] * SB->nsq[q] >= eps2) r |= 1u << p;
+    dropped += __builtin_popcount(am) - __builtin_popcount(r);
+  }
+  // columns grouped by their kept-row set; each group is one entry
+  uint32_t remaining = bm;
+  while (remaining) {
+    const int q = __builtin_ctz(remaining);
+    uint32_t r = 0;
+    for (int p = 0; p < kBandAtoms; ++p)
+      if (((am >> p) & 1u) && SA->nsq[p] * SB->nsq[q] >= eps2) r |= 1u << p;
+    uint32_t cm = 0;
+    for (int q2 = q; q2 < kBandAtoms; ++q2) {
+      if (!((remaining >> q2) & 1u)) continue;
+      uint32_t r2 = 0;
+      for (int p = 0; p < kBandAtoms; ++p)
+        if (((am >> p) & 1u) && SA->nsq[p] * SB->nsq[q2] >= eps2) r2 |= 1u << p;
+      if (r2 == r) cm |= 1u << q2;
+    }
+    remaining &= ~cm;
+    if (r) emit(r, cm);
+  }
+  return dropped;
+}
+
+__global__ __launch_bounds__(64) void sym_count(
+    int32_t nbands, int nseg, const Band* __restrict__ bands, const int32_t* __restrict__ blk,
+    const Strip* __restrict__ as, const int64_t* __restrict__ a_ptr,
+    const Strip* __restrict__ bs, const int64_t* __restrict__ b_ptr, double eps2,
+    const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
+    const int32_t* __restrict__ band_of, const int32_t* __restrict__ pos_in_band,
+    int32_t* __restrict__ cnt, int32_t* __restrict__ ksum, uint32_t* __restrict__ pat,
+    unsigned long long* __restrict__ stats) {
+  const int64_t w = blockIdx.x;
+  const int32_t I = (int32_t)(w / nseg);
+  const int g = (int)(w % nseg);
+  if (I >= nbands) return;
+  const int lane = threadIdx.x;
+  const int64_t sa0 = a_ptr[I], sa1 = a_ptr[I + 1];
+  const int64_t len = sa1 - sa0;
+  const int64_t lo = sa0 + len * g / nseg, hi = sa0 + len * (g + 1) / nseg;
+  int32_t* c = cnt + ((int64_t)I * nseg + g) * nbands;
+  int32_t* ks = ksum + ((int64_t)I * nseg + g) * nbands;
+  uint32_t* pt = pat + (int64_t)I * nbands * (kBandAtoms / 2);
+  __shared__ int16_t rsI[kBandAtoms + 1];
+  if (lane <= kBandAtoms) rsI[lane] = bands[I].rs[min(lane, bands[I].natoms)];
+  __syncthreads();
+  unsigned long long kept = 0, dropped = 0;
+  double flops = 0.0;
+  for (int64_t sa = lo; sa < hi; ++sa) {
+    const Strip* SA = as + sa;
+    const int32_t k = SA->k;
+    const int kd = blk[k];
+    const int64_t b0 = b_ptr[k], b1 = b_ptr[k + 1];
+    for (int64_t sb = b0 + lane; sb < b1; sb += 64) {
+      const Strip* SB = bs + sb;
+      const int32_t J = SB->band;
+      int n = 0;
+      dropped += pair_entries(SA, SB, eps2, [&](uint32_t rm, uint32_t cm) {
+        // a zero-size K atom adds to the C pattern but holds no entry
+        if (kd > 0) ++n;
+        kept += (unsigned long long)__builtin_popcount(rm) * __builtin_popcount(cm);
+        // rows of cm in band J
+        int cr = 0;
+        for (int q = 0; q < kBandAtoms; ++q)
+          if ((cm >> q) & 1u) cr += bands[J].rs[q + 1] - bands[J].rs[q];
+        flops += 2.0 * rows_of_mask(rsI, rm) * kd * cr;
+        for (int p = 0; p < kBandAtoms; ++p)
+          if ((rm >> p) & 1u) atomicOr(&pt[(int64_t)J * (kBandAtoms / 2) + (p >> 1)], cm << (16 * (p & 1)));
+      });
+      if (n) {
+        atomicAdd(&c[J], n);
+        atomicAdd(&ks[J], n * kd);
+      }
+    }
+  }
+  if (g == 0 && old_row_p != nullptr) {
+    const Band& bI = bands[I];
+    for (int p = 0; p < bI.natoms; ++p) {
+      const int32_t i = bI.first + p;
+      for (int64_t b = old_row_p[i] + lane; b < old_row_p[i + 1]; b += 64) {
+        const int32_t j = old_col[b];
+        const int32_t J = band_of[j];
+        atomicOr(&pt[(int64_t)J * (kBandAtoms / 2) + (p >> 1)],
+                 (1u << pos_in_band[j]) << (16 * (p & 1)));
+      }
+    }
+  }
+  // stats: [0] kept triples, [1] dropped, [2] flops (as integer: exact below 2^63)
+  for (int o = 32; o > 0; o >>= 1) {
+    kept += __shfl_xor(kept, o, 64);
+    dropped += __shfl_xor(dropped, o, 64);
+    flops += __shfl_xor(flops, o, 64);
+  }
+  if (lane == 0) {
+    atomicAdd(&stats[0], kept);
+    atomicAdd(&stats[1], dropped);
+    atomicAdd(&stats[2], (unsigned long long)flops);
+  }
+}
+
+// Per band: tiles present, their entry counts and C sizes (lanes over J).
+__global__ __launch_bounds__(64) void sym_band_totals(
+    int32_t nbands, int nseg, const Band* __restrict__ bands, const int32_t* __restrict__ cnt,
+    const uint32_t* __restrict__ pat, int64_t* __restrict__ band_tiles,
+    int64_t* __restrict__ band_entries, int64_t* __restrict__ band_csize,
+    int64_t* __restrict__ band_cblocks) {
+  const int32_t I = blockIdx.x;
+  if (I >= nbands) return;
+  const int lane = threadIdx.x;
+  const Band& bI = bands[I];
+  int64_t nt = 0, ne = 0, cs = 0, cb = 0;
+  for (int32_t J = lane; J < nbands; J += 64) {
+    int64_t e = 0;
+    for (int g = 0; g < nseg; ++g) e += cnt[((int64_t)I * nseg + g) * nbands + J];
+    const uint32_t* pt = pat + ((int64_t)I * nbands + J) * (kBandAtoms / 2);
+    bool any = false;
+    const Band& bJ = bands[J];
+    for (int p = 0; p < bI.natoms; ++p) {
+      const uint32_t m = (pt[p >> 1] >> (16 * (p & 1))) & 0xFFFFu;
+      if (!m) continue;
+      any = true;
+      const int mr = bI.rs[p + 1] - bI.rs[p];
+      for (int q = 0; q < bJ.natoms; ++q)
+        if ((m >> q) & 1u) {
+          cs += (int64_t)mr * (bJ.rs[q + 1] - bJ.rs[q]);
+          ++cb;
+        }
+    }
+    if (any || e) {
+      ++nt;
+      ne += e;
+    }
+  }
+  nt = wave_sum_i64(nt);
+  ne = wave_sum_i64(ne);
+  cs = wave_sum_i64(cs);
+  cb = wave_sum_i64(cb);
+  if (lane == 0) {
+    band_tiles[I] = nt;
+    band_entries[I] = ne;
+    band_csize[I] = cs;
+    band_cblocks[I] = cb;
+  }
+}
+
+// Tile descriptors (J ascending within a band); turns cnt/ksum into the
+// per-(I, g, J) entry cursors and K-column starts used by sym_fill.
+__global__ __launch_bounds__(64) void sym_tiles(
+    int32_t nbands, int nseg, const Band* __restrict__ bands, int32_t* __restrict__ cnt,
+    int32_t* __restrict__ ksum, const uint32_t* __restrict__ pat,
+    const int64_t* __restrict__ tile_base, const int64_t* __restrict__ entry_base,
+    const int64_t* __restrict__ csize_base, TileDesc* __restrict__ tiles) {
+  const int32_t I = blockIdx.x;
+  if (I >= nbands) return;
+  const int lane = threadIdx.x;
+  const Band& bI = bands[I];
+  int64_t t_run = tile_base[I], e_run = entry_base[I], c_run = csize_base[I];
+  for (int32_t J0 = 0; J0 < nbands; J0 += 64) {
+    const int32_t J = J0 + lane;
+    int64_t e = 0, k = 0, cs = 0;
+    bool any = false;
+    uint32_t pw[kBandAtoms / 2] = {};
+    if (J < nbands) {
+      for (int g = 0; g < nseg; ++g) {
+        e += cnt[((int64_t)I * nseg + g) * nbands + J];
+        k += ksum[((int64_t)I * nseg + g) * nbands + J];
+      }
+      const uint32_t* pt = pat + ((int64_t)I * nbands + J) * (kBandAtoms / 2);
+      const Band& bJ = bands[J];
+      for (int w = 0; w < kBandAtoms / 2; ++w) pw[w] = pt[w];
+      for (int p = 0; p < bI.natoms; ++p) {
+        const uint32_t m = (pw[p >> 1] >> (16 * (p & 1))) & 0xFFFFu;
+        if (!m) continue;
+        any = true;
+        const int mr = bI.rs[p + 1] - bI.rs[p];
+        for (int q = 0; q < bJ.natoms; ++q)
+          if ((m >> q) & 1u) cs += (int64_t)mr * (bJ.rs[q + 1] - bJ.rs[q]);
+      }
+    }
+    const bool present = any || e > 0;
+    const int64_t t_off = wave_excl_i64(present ? 1 : 0);
+    const int64_t e_off = wave_excl_i64(present ? e : 0);
+    const int64_t c_off = wave_excl_i64(present ? cs : 0);
+    if (present) {
+      TileDesc td;
+      td.I = I;
+      td.J = J;
+      td.e0 = (int32_t)(e_run + e_off);
+      td.ne = (int32_t)e;
+      td.ktot = (int32_t)k;
+      td.pad = 0;
+      td.out_base = c_run + c_off;
+      for (int p = 0; p < kBandAtoms; ++p)
+        td.pattern[p] = (uint16_t)((pw[p >> 1] >> (16 * (p & 1))) & 0xFFFFu);
+      tiles[t_run + t_off] = td;
+      // segment cursors: entries / K columns of earlier segments come first
+      int64_t ecur = e_run + e_off, kcur = 0;
+      for (int g = 0; g < nseg; ++g) {
+        const int64_t idx = ((int64_t)I * nseg + g) * nbands + J;
+        const int32_t ce = cnt[idx], ck = ksum[idx];
+        cnt[idx] = (int32_t)ecur;
+        ksum[idx] = (int32_t)kcur;
+        ecur += ce;
+        kcur += ck;
+      }
+    }
+    t_run += wave_sum_i64(present ? 1 : 0);
+    e_run += wave_sum_i64(present ? e : 0);
+    c_run += wave_sum_i64(present ? cs : 0);
+  }
+}
+
+__global__ __launch_bounds__(64) void sym_fill(
+    int32_t nbands, int nseg, const int32_t* __restrict__ blk, const Strip* __restrict__ as,
+    const int64_t* __restrict__ a_ptr, const Strip* __restrict__ bs,
+    const int64_t* __restrict__ b_ptr, const int64_t* __restrict__ a_pk,
+    const int64_t* __restrict__ b_pk, double eps2, int32_t* __restrict__ cursor,
+    int32_t* __restrict__ kcur, Entry* __restrict__ entries) {
+  const int64_t w = blockIdx.x;
+  const int32_t I = (int32_t)(w / nseg);
+  const int g = (int)(w % nseg);
+  if (I >= nbands) return;
+  const int lane = threadIdx.x;
+  const int64_t sa0 = a_ptr[I], sa1 = a_ptr[I + 1];
+  const int64_t len = sa1 - sa0;
+  const int64_t lo = sa0 + len * g / nseg, hi = sa0 + len * (g + 1) / nseg;
+  int32_t* c = cursor + ((int64_t)I * nseg + g) * nbands;
+  int32_t* kc = kcur + ((int64_t)I * nseg + g) * nbands;
+  for (int64_t sa = lo; sa < hi; ++sa) {
+    const Strip* SA = as + sa;
+    const int32_t k = SA->k;
+    const int kd = blk[k];
+    const int64_t b0 = b_ptr[k], b1 = b_ptr[k + 1];
+    for (int64_t sb = b0 + lane; sb < b1; sb += 64) {
+      const Strip* SB = bs + sb;
+      const int32_t J = SB->band;
+      if (kd <= 0) continue;
+      int n = 0;
+      pair_entries(SA, SB, eps2, [&](uint32_t, uint32_t) { ++n; });
+      if (!n) continue;
+      // reserve n entry slots and n*kd K columns of tile (I, J); the lanes
+      // of one step hold distinct J and steps run in K order
+      int32_t e = atomicAdd(&c[J], n);
+      int32_t kk = atomicAdd(&kc[J], n * kd);
+      pair_entries(SA, SB, eps2, [&](uint32_t rm, uint32_t cm) {
+        Entry en;
+        en.a_col = (uint32_t)(a_pk[sa] / 64);
+        en.b_col = (uint32_t)(b_pk[sb] / 64);
+        en.kstart = kk;
+        en.rmask = (uint16_t)rm;
+        en.cmask = (uint16_t)cm;
+        entries[e] = en;
+        ++e;
+        kk += kd;
+      });
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// C index from the numeric kernel's kept masks. One wave per row band I,
+// lanes 0..15 = row positions; the band's tiles are visited in J order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void cidx_count(int32_t nbands, const Band* __restrict__ bands,
+                                                 const int64_t* __restrict__ tile_base,
+                                                 const uint16_t* __restrict__ kept,
+                                                 int64_t* __restrict__ row_cnt) {
+  const int32_t I = blockIdx.x;
+  if (I >= nbands) return;
+  const int lane = threadIdx.x;
+  const int nat = bands[I].natoms;
+  int64_t n = 0;
+  if (lane < nat)
+    for (int64_t t = tile_base[I]; t < tile_base[I + 1]; ++t)
+      n += __builtin_popcount(kept[t * kBandAtoms + lane]);
+  if (lane < nat) row_cnt[bands[I].first + lane] = n;
+}
+
+__global__ __launch_bounds__(64) void cidx_fill(int32_t nbands, const Band* __restrict__ bands,
+                                                const int64_t* __restrict__ tile_base,
+                                                const TileDesc* __restrict__ tiles,
+                                                const uint16_t* __restrict__ kept,
+                                                const int64_t* __restrict__ row_p,
+                                                int32_t* __restrict__ col, int64_t* __restrict__ off,
+                                                int32_t* __restrict__ rows,
+                                                unsigned long long* __restrict__ nelems) {
+  const int32_t I = blockIdx.x;
+  if (I >= nbands) return;
+  const int lane = threadIdx.x;
+  const Band& bI = bands[I];
+  const int p = lane;
+  const bool act = p < bI.natoms;
+  const int m = act ? bI.rs[p + 1] - bI.rs[p] : 0;
+  int64_t b = act ? row_p[bI.first + p] : 0;
+  unsigned long long ne = 0;
+  for (int64_t t = tile_base[I]; t < tile_base[I + 1]; ++t) {
+    const TileDesc* td = tiles + t;
+    const Band& bJ = bands[td->J];
+    const uint32_t pm = act ? td->pattern[p] : 0u;
+    int64_t rowsz = 0;
+    for (int q = 0; q < bJ.natoms; ++q)
+      if ((pm >> q) & 1u) rowsz += (int64_t)m * (bJ.rs[q + 1] - bJ.rs[q]);
+    // exclusive prefix over the 16 row positions
+    int64_t x = rowsz;
+    for (int o = 1; o < 16; o <<= 1) {
+      const int64_t y = __shfl_up(x, o, 64);
+      if (p >= o) x += y;
+    }
+    int64_t o_run = td->out_base + x - rowsz;
+    const uint32_t km = act ? kept[t * kBandAtoms + p] : 0u;
+    for (int q = 0; q < bJ.natoms; ++q) {
+      if (!((pm >> q) & 1u)) continue;
+      const int64_t sz = (int64_t)m * (bJ.rs[q + 1] - bJ.rs[q]);
+      if ((km >> q) & 1u) {
+        col[b] = bJ.first + q;
+        off[b] = o_run;
+        rows[b] = bI.first + p;
+        ++b;
+        ne += (unsigned long long)sz;
+      }
+      o_run += sz;
+    }
+  }
+  ne = wave_sum_i64((int64_t)ne);
+  if (lane == 0) atomicAdd(nelems, ne);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+#define CK(expr)                             \
+  do {                                       \
+    hipError_t e_ = (expr);                  \
+    if (e_ != hipSuccess) return e_;         \
+  } while (0)
+
+static unsigned grid_of(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+hipError_t build_strips(hipStream_t s, SymScratch& W, const MatView& M, const DistView& D,
+                        bool a_side, StripSet& out) {
+  const int64_t nblk = M.nblk;
+  // the slot sequence: merged (A) or CSR (B) order
+  CK(W.flag.ensure(sizeof(int64_t) * (size_t)(nblk + 1)));
+  CK(W.excl.ensure(sizeof(int64_t) * (size_t)(nblk + 2)));
+  CK(W.scan.ensure(sizeof(int64_t) * (size_t)scan_scratch_elems(nblk + 1)));
+  if (a_side) {
+    CK(W.merged.ensure(sizeof(int64_t) * (size_t)std::max<int64_t>(nblk, 1)));
+    if (nblk)
+      hipLaunchKernelGGL(a_merge_rank, dim3(grid_of(nblk)), dim3(256), 0, s, nblk, M.rows, M.col,
+                         M.row_p, D.band_of, D.pos_in_band, D.bands, (int64_t*)W.merged.p);
+  }
+  if (nblk)
+    hipLaunchKernelGGL(strip_flags, dim3(grid_of(nblk)), dim3(256), 0, s, nblk, a_side ? 1 : 0,
+                       (const int64_t*)W.merged.p, M.rows, M.col, D.band_of,
+                       (int64_t*)W.flag.p);
+  CK(hipGetLastError());
+  CK(exclusive_scan(s, (const int64_t*)W.flag.p, (int64_t*)W.excl.p, nblk, (int64_t*)W.scan.p));
+  int64_t nstrips = 0;
+  CK(hipMemcpyAsync(&nstrips, (int64_t*)W.excl.p + nblk, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  CK(hipStreamSynchronize(s));
+  out.n = nstrips;
+  CK(out.strips.ensure(sizeof(Strip) * (size_t)std::max<int64_t>(nstrips, 1)));
+  if (nstrips)
+    hipLaunchKernelGGL(strip_init, dim3(grid_of(nstrips)), dim3(256), 0, s, nstrips,
+                       (Strip*)out.strips.p);
+  if (nblk)
+    hipLaunchKernelGGL(strip_fill, dim3(grid_of(nblk)), dim3(256), 0, s, nblk, a_side ? 1 : 0,
+                       (const int64_t*)W.merged.p, (const int64_t*)W.excl.p,
+                       (const int64_t*)W.flag.p, M.rows, M.col, M.off, M.nsq, D.band_of,
+                       D.pos_in_band, (Strip*)out.strips.p);
+  CK(hipGetLastError());
+  // group pointers: per band (A) or per row atom (B)
+  const int64_t ngroups = a_side ? D.nbands : D.n;
+  CK(out.ptr.ensure(sizeof(int64_t) * (size_t)(ngroups + 1)));
+  CK(W.gslot.ensure(sizeof(int64_t) * (size_t)std::max<int64_t>(ngroups, 1)));
+  if (a_side) {
+    if (D.nbands)
+      hipLaunchKernelGGL(band_first_slot, dim3(grid_of(D.nbands)), dim3(256), 0, s, D.nbands,
+                         D.bands, M.row_p, (int64_t*)W.gslot.p);
+  } else {
+    if (D.n)
+      CK(hipMemcpyAsync(W.gslot.p, M.row_p, sizeof(int64_t) * (size_t)D.n,
+                        hipMemcpyDeviceToDevice, s));
+  }
+  hipLaunchKernelGGL(group_ptr, dim3(grid_of(ngroups + 1)), dim3(256), 0, s, ngroups,
+                     (const int64_t*)W.gslot.p, (const int64_t*)W.excl.p, nblk, nstrips,
+                     (int64_t*)out.ptr.p);
+  CK(hipGetLastError());
+  // packed panel offsets
+  CK(out.pk.ensure(sizeof(int64_t) * (size_t)(nstrips + 1)));
+  CK(W.flag.ensure(sizeof(int64_t) * (size_t)(std::max(nblk, nstrips) + 1)));
+  CK(W.scan.ensure(sizeof(int64_t) * (size_t)scan_scratch_elems(std::max(nblk, nstrips) + 1)));
+  if (nstrips)
+    hipLaunchKernelGGL(strip_panel_size, dim3(grid_of(nstrips)), dim3(256), 0, s, nstrips,
+                       (const Strip*)out.strips.p, D.blk, (int64_t*)W.flag.p);
+  CK(exclusive_scan(s, (const int64_t*)W.flag.p, (int64_t*)out.pk.p, nstrips, (int64_t*)W.scan.p));
+  CK(hipMemcpyAsync(&out.pk_total, (int64_t*)out.pk.p + nstrips, sizeof(int64_t),
+                    hipMemcpyDeviceToHost, s));
+  CK(hipStreamSynchronize(s));
+  return hipSuccess;
+}
+
+hipError_t build_tiles(hipStream_t s, SymScratch& W, const DistView& D, const StripSet& A,
+                       const StripSet& B, const MatView* Cold, double eps2, SymResult& r) {
+  const int32_t nb = D.nbands;
+  const int nseg = 8;
+  const int64_t dense = (int64_t)nb * nseg * nb;
+  CK(W.cnt.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(dense, 1)));
+  CK(W.ksum.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(dense, 1)));
+  CK(W.pat.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>((int64_t)nb * nb * (kBandAtoms / 2), 1)));
+  CK(W.stats.ensure(sizeof(unsigned long long) * 4));
+  CK(fill32(s, (int32_t*)W.cnt.p, dense, 0));
+  CK(fill32(s, (int32_t*)W.ksum.p, dense, 0));
+  CK(fill32(s, (int32_t*)W.pat.p, (int64_t)nb * nb * (kBandAtoms / 2), 0));
+  CK(fill32(s, (int32_t*)W.stats.p, 8, 0));
+  if (nb)
+    hipLaunchKernelGGL(sym_count, dim3((unsigned)(nb * nseg)), dim3(64), 0, s, nb, nseg, D.bands,
+                       D.blk, (const Strip*)A.strips.p, (const int64_t*)A.ptr.p,
+                       (const Strip*)B.strips.p, (const int64_t*)B.ptr.p, eps2,
+                       Cold ? Cold->row_p : nullptr, Cold ? Cold->col : nullptr, D.band_of,
+                       D.pos_in_band, (int32_t*)W.cnt.p, (int32_t*)W.ksum.p,
+                       (uint32_t*)W.pat.p, (unsigned long long*)W.stats.p);
+  CK(hipGetLastError());
+  // per-band totals and their scans
+  CK(W.band4.ensure(sizeof(int64_t) * 4 * (size_t)(nb + 1)));
+  CK(W.band4s.ensure(sizeof(int64_t) * 4 * (size_t)(nb + 2)));
+  int64_t* bt = (int64_t*)W.band4.p;
+  int64_t* bts = (int64_t*)W.band4s.p;
+  if (nb)
+    hipLaunchKernelGGL(sym_band_totals, dim3(nb), dim3(64), 0, s, nb, nseg, D.bands,
+                       (const int32_t*)W.cnt.p, (const uint32_t*)W.pat.p, bt, bt + (nb + 1),
+                       bt + 2 * (nb + 1), bt + 3 * (nb + 1));
+  CK(hipGetLastError());
+  CK(W.scan.ensure(sizeof(int64_t) * (size_t)scan_scratch_elems(nb + 1)));
+  for (int v = 0; v < 4; ++v)
+    CK(exclusive_scan(s, bt + v * (nb + 1), bts + v * (nb + 2), nb, (int64_t*)W.scan.p));
+  int64_t tot[4];
+  unsigned long long st[3];
+  for (int v = 0; v < 4; ++v)
+    CK(hipMemcpyAsync(&tot[v], bts + v * (nb + 2) + nb, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  CK(hipMemcpyAsync(st, W.stats.p, sizeof(st), hipMemcpyDeviceToHost, s));
+  CK(hipStreamSynchronize(s));
+  r.n_tiles = tot[0];
+  r.n_entries = tot[1];
+  r.c_size = tot[2];
+  r.n_struct = tot[3];
+  r.kept = (int64_t)st[0];
+  r.dropped = (int64_t)st[1];
+  r.flops = (double)st[2];
+  CK(r.tiles.ensure(sizeof(TileDesc) * (size_t)std::max<int64_t>(r.n_tiles, 1)));
+  CK(r.entries.ensure(sizeof(Entry) * (size_t)std::max<int64_t>(r.n_entries, 1)));
+  CK(r.tile_base.ensure(sizeof(int64_t) * (size_t)(nb + 1)));
+  CK(hipMemcpyAsync(r.tile_base.p, bts, sizeof(int64_t) * (size_t)(nb + 1), hipMemcpyDeviceToDevice, s));
+  if (nb)
+    hipLaunchKernelGGL(sym_tiles, dim3(nb), dim3(64), 0, s, nb, nseg, D.bands, (int32_t*)W.cnt.p,
+                       (int32_t*)W.ksum.p, (const uint32_t*)W.pat.p, bts, bts + (nb + 2),
+                       bts + 2 * (nb + 2), (TileDesc*)r.tiles.p);
+  CK(hipGetLastError());
+  if (nb)
+    hipLaunchKernelGGL(sym_fill, dim3((unsigned)(nb * nseg)), dim3(64), 0, s, nb, nseg, D.blk,
+                       (const Strip*)A.strips.p, (const int64_t*)A.ptr.p,
+                       (const Strip*)B.strips.p, (const int64_t*)B.ptr.p,
+                       (const int64_t*)A.pk.p, (const int64_t*)B.pk.p, eps2, (int32_t*)W.cnt.p,
+                       (int32_t*)W.ksum.p, (Entry*)r.entries.p);
+  return hipGetLastError();
+}
+
+hipError_t build_c_index(hipStream_t s, SymScratch& W, const DistView& D, const SymResult& r,
+                         const uint16_t* kept, CIndex& c) {
+  const int32_t nb = D.nbands;
+  CK(c.row_p.ensure(sizeof(int64_t) * (size_t)(D.n + 1)));
+  CK(W.flag.ensure(sizeof(int64_t) * (size_t)(D.n + 1)));
+  CK(W.scan.ensure(sizeof(int64_t) * (size_t)scan_scratch_elems(D.n + 1)));
+  CK(W.stats.ensure(sizeof(unsigned long long) * 4));
+  if (nb)
+    hipLaunchKernelGGL(cidx_count, dim3(nb), dim3(64), 0, s, nb, D.bands,
+                       (const int64_t*)r.tile_base.p, kept, (int64_t*)W.flag.p);
+  CK(hipGetLastError());
+  CK(exclusive_scan(s, (const int64_t*)W.flag.p, (int64_t*)c.row_p.p, D.n, (int64_t*)W.scan.p));
+  int64_t nblk = 0;
+  CK(hipMemcpyAsync(&nblk, (int64_t*)c.row_p.p + D.n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  CK(hipStreamSynchronize(s));
+  c.nblk = nblk;
+  CK(c.col.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(nblk, 1)));
+  CK(c.off.ensure(sizeof(int64_t) * (size_t)std::max<int64_t>(nblk, 1)));
+  CK(c.rows.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(nblk, 1)));
+  CK(fill32(s, (int32_t*)W.stats.p, 2, 0));
+  if (nb)
+    hipLaunchKernelGGL(cidx_fill, dim3(nb), dim3(64), 0, s, nb, D.bands,
+                       (const int64_t*)r.tile_base.p, (const TileDesc*)r.tiles.p, kept,
+                       (const int64_t*)c.row_p.p, (int32_t*)c.col.p, (int64_t*)c.off.p,
+                       (int32_t*)c.rows.p, (unsigned long long*)W.stats.p);
+  CK(hipGetLastError());
+  unsigned long long ne = 0;
+  CK(hipMemcpyAsync(&ne, W.stats.p, sizeof(ne), hipMemcpyDeviceToHost, s));
+  CK(hipStreamSynchronize(s));
+  c.nelems = (int64_t)ne;
+  return hipSuccess;
+}
+
+}  // namespace lamb
