@@ -108,20 +108,10 @@ hipError_t launch_pack_strips(hipStream_t s, int64_t n_strips, const Strip* stri
 // ---------------------------------------------------------------------------
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-constexpr int kEntryBatch = 256;  // entries staged in LDS at a time
-constexpr int kPrefetch = 4;      // K steps in flight per wave
-
-struct LdsEntry {
-  uint32_t a_col, b_col;  // first packed panel column of the A / B strip
-  uint32_t kd;            // K columns of the entry
-  uint32_t masks;         // rmask | cmask << 16
-};
-
-// One K column of a wave's stream (wave-uniform).
-struct KCol {
-  int64_t a, b;  // panel column start (element offset), -1 = none
-  uint32_t masks;
-};
+// K columns of the per-tile step table held in LDS at a time
+constexpr int kTableCols = 2048;
+constexpr int kPrefetch = 4;  // K steps in flight per wave
+constexpr uint32_t kNoCol = 0xFFFFFFFFu;
 
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -137,11 +127,16 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_kernel(
     const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
     const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
     double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
-  // main loop: the staged entry batch; epilogue: the C tile [64][kPitch]
+  // main loop: the step table of a window of K columns; epilogue: C tile
   __shared__ double lds[kBandRows * kPitch];
-  static_assert(sizeof(LdsEntry) * kEntryBatch <= sizeof(double) * kBandRows * kPitch, "");
-  LdsEntry* ent = reinterpret_cast<LdsEntry*>(lds);
+  uint32_t* col_a = reinterpret_cast<uint32_t*>(lds);  // packed A panel column
+  uint32_t* col_b = col_a + kTableCols;                // packed B panel column
+  uint32_t* col_m = col_b + kTableCols;                // rmask | cmask << 16
+  uint32_t* step_g = col_m + kTableCols;               // row groups | col groups << 16
+  static_assert(sizeof(uint32_t) * (3 * kTableCols + kTableCols / 4) <=
+                    sizeof(double) * kBandRows * kPitch, "step table exceeds LDS buffer");
   __shared__ int16_t rs_r[kBandAtoms + 1], rs_c[kBandAtoms + 1];
+  __shared__ uint16_t pg_r[kBandAtoms], pg_c[kBandAtoms];
   __shared__ int32_t pair_id[kBandAtoms * kBandAtoms];
   __shared__ int64_t pair_off[kBandAtoms * kBandAtoms];
   __shared__ int32_t wave_cnt[4];
@@ -171,31 +166,16 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_kernel(
   }
   if (tid < kBandAtoms) keptw[tid] = 0;
   __syncthreads();
-
-  // atoms overlapping this wave's 16-row halves / 16-column quads
-  uint32_t hm_r[2] = {0u, 0u}, hm_c[2] = {0u, 0u};
-  for (int a = 0; a < natI; ++a) {
-    const int r0 = rs_r[a], r1 = rs_r[a + 1];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int lo = 32 * wr + 16 * h;
-      if (r1 > r0 && r0 < lo + 16 && r1 > lo) hm_r[h] |= 1u << a;
-    }
+  if (tid < kBandAtoms) {
+    uint32_t g = 0;
+    if (tid < natI && rs_r[tid + 1] > rs_r[tid])
+      for (int q = rs_r[tid] / 4; q <= (rs_r[tid + 1] - 1) / 4; ++q) g |= 1u << q;
+    pg_r[tid] = (uint16_t)g;
+    g = 0;
+    if (tid < natJ && rs_c[tid + 1] > rs_c[tid])
+      for (int q = rs_c[tid] / 4; q <= (rs_c[tid + 1] - 1) / 4; ++q) g |= 1u << q;
+    pg_c[tid] = (uint16_t)g;
   }
-  for (int a = 0; a < natJ; ++a) {
-    const int c0 = rs_c[a], c1 = rs_c[a + 1];
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int lo = 32 * wc + 16 * q;
-      if (c1 > c0 && c0 < lo + 16 && c1 > lo) hm_c[q] |= 1u << a;
-    }
-  }
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    hm_r[h] = uni(hm_r[h]);
-    hm_c[h] = uni(hm_c[h]);
-  }
-  const uint32_t wrows = hm_r[0] | hm_r[1], wcols = hm_c[0] | hm_c[1];
   // band position of this lane's operand rows (A) and columns (B)
   int rpos[2] = {-1, -1}, cpos[2] = {-1, -1};
   for (int a = 0; a < natI; ++a)
@@ -211,121 +191,97 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_kernel(
       if (c >= rs_c[a] && c < rs_c[a + 1]) cpos[q] = a;
     }
   const int kc = lane >> 4;  // K column of this lane within a step
+  // the 16-bit row / column group masks (4 rows or columns per group) of
+  // this wave's halves and quads: rows 32wr+16h.., cols 32wc+16q..
+  const uint32_t gsh_r = 8 * wr, gsh_c = 8 * wc;
 
   dbl4 acc[4];
 #pragma unroll
   for (int x = 0; x < 4; ++x) acc[x] = dbl4{0.0, 0.0, 0.0, 0.0};
 
-  for (int b0 = 0; b0 < ne; b0 += kEntryBatch) {
-    const int nbatch = min(kEntryBatch, ne - b0);
-    if (b0 > 0) __syncthreads();  // every wave is done with the previous batch
-    if (tid < nbatch) {
-      const int e = e0 + b0 + tid;
-      const Entry en = entries[e];
-      const int ks_next = (b0 + tid + 1 < ne) ? entries[e + 1].kstart : ktot;
-      LdsEntry le;
-      le.a_col = en.a_col;
-      le.b_col = en.b_col;
-      le.kd = (uint32_t)(ks_next - en.kstart);
-      le.masks = (uint32_t)en.rmask | ((uint32_t)en.cmask << 16);
-      ent[tid] = le;
+  for (int Q0 = 0; Q0 < ktot; Q0 += kTableCols) {
+    const int ncols = min(kTableCols, ktot - Q0);
+    const int nsteps = (ncols + 3) >> 2;
+    __syncthreads();  // previous window done (and pg_r / pg_c visible)
+    for (int t = tid; t < ((ncols + 3) & ~3); t += 256) col_a[t] = kNoCol;
+    for (int t = tid; t < nsteps; t += 256) step_g[t] = 0;
+    __syncthreads();
+    // one thread per entry writes its columns that fall in this window
+    for (int e = tid; e < ne; e += 256) {
+      const Entry en = entries[e0 + e];
+      const int ks = en.kstart;
+      const int ke = (e + 1 < ne) ? entries[e0 + e + 1].kstart : ktot;
+      const int lo = max(ks, Q0), hi = min(ke, Q0 + ncols);
+      if (lo >= hi) continue;
+      uint32_t gr = 0, gc = 0;
+      for (int p = 0; p < kBandAtoms; ++p) {
+        if ((en.rmask >> p) & 1u) gr |= pg_r[p];
+        if ((en.cmask >> p) & 1u) gc |= pg_c[p];
+      }
+      const uint32_t m = (uint32_t)en.rmask | ((uint32_t)en.cmask << 16);
+      const uint32_t g = gr | (gc << 16);
+      for (int Q = lo; Q < hi; ++Q) {
+        const int q = Q - Q0;
+        col_a[q] = en.a_col + (uint32_t)(Q - ks);
+        col_b[q] = en.b_col + (uint32_t)(Q - ks);
+        col_m[q] = m;
+      }
+      for (int st = (lo - Q0) >> 2; st <= (hi - 1 - Q0) >> 2; ++st) atomicOr(&step_g[st], g);
     }
     __syncthreads();
 
-    // ---- per-wave stream over this batch ----
-    int ei = 0;        // next entry of the batch
-    uint32_t kk = 0, kd = 0, acol = 0, bcol = 0, msk = 0;
-    // next relevant column of the stream (wave-uniform), or a = -1
-    auto next_col = [&]() -> KCol {
-      while (kk >= kd && ei < nbatch) {
-        const LdsEntry le = ent[ei];
-        ++ei;
-        acol = uni(le.a_col);
-        bcol = uni(le.b_col);
-        kd = uni(le.kd);
-        msk = uni(le.masks);
-        kk = ((msk & 0xFFFFu) & wrows) && ((msk >> 16) & wcols) ? 0u : kd;
-      }
-      KCol col;
-      if (kk < kd) {
-        col.a = (int64_t)(acol + kk) * 64;
-        col.b = (int64_t)(bcol + kk) * 64;
-        col.masks = msk;
-        ++kk;
-      } else {
-        col.a = -1;
-        col.b = -1;
-        col.masks = 0;
-      }
-      return col;
-    };
-
-    // ring of kPrefetch steps: operands + which 16x16 blocks run
+    // ---- per-wave walk over the window's steps, kPrefetch steps ahead ----
+    int snext = 0;  // next step to examine
     double ra[kPrefetch][2], rb[kPrefetch][2];
-    uint32_t blkmask[kPrefetch];  // bit 2h+q: block (h, q) runs
-    bool more = true;
-    // Assemble the next step and issue its loads into ring slot S.
-#define LAMB_FETCH(S)                                                           \
-    {                                                                           \
-      KCol c0 = next_col(), c1 = next_col(), c2 = next_col(), c3 = next_col(); \
-      more = c3.a >= 0;                                                         \
-      uint32_t hb = 0, qb = 0;                                                  \
-      const uint32_t m0 = c0.masks, m1 = c1.masks, m2 = c2.masks, m3 = c3.masks; \
-      const uint32_t mall = m0 | m1 | m2 | m3;                                  \
-      _Pragma("unroll") for (int h = 0; h < 2; ++h)                             \
-        if (mall & hm_r[h]) hb |= 1u << h;                                      \
-      _Pragma("unroll") for (int q = 0; q < 2; ++q)                             \
-        if ((mall >> 16) & hm_c[q]) qb |= 1u << q;                              \
-      uint32_t bm = 0;                                                          \
-      _Pragma("unroll") for (int h = 0; h < 2; ++h)                             \
-        _Pragma("unroll") for (int q = 0; q < 2; ++q)                           \
-          if (((hb >> h) & 1u) && ((qb >> q) & 1u)) bm |= 1u << (2 * h + q);    \
-      blkmask[S] = bm;                                                          \
-      const int64_t la = kc == 0 ? c0.a : kc == 1 ? c1.a : kc == 2 ? c2.a : c3.a; \
-      const int64_t lb = kc == 0 ? c0.b : kc == 1 ? c1.b : kc == 2 ? c2.b : c3.b; \
-      const uint32_t lm = kc == 0 ? m0 : kc == 1 ? m1 : kc == 2 ? m2 : m3;     \
-      _Pragma("unroll") for (int h = 0; h < 2; ++h) {                           \
-        double v = 0.0;                                                         \
-        if ((bm & (3u << (2 * h))) && la >= 0 && rpos[h] >= 0 && ((lm >> rpos[h]) & 1u)) \
-          v = a_pack[la + 32 * wr + 16 * h + (lane & 15)];                      \
-        ra[S][h] = v;                                                           \
-      }                                                                         \
-      _Pragma("unroll") for (int q = 0; q < 2; ++q) {                           \
-        double v = 0.0;                                                         \
-        if ((bm & (5u << q)) && lb >= 0 && cpos[q] >= 0 && ((lm >> (16 + cpos[q])) & 1u)) \
-          v = b_pack[lb + 32 * wc + 16 * q + (lane & 15)];                      \
-        rb[S][q] = v;                                                           \
-      }                                                                         \
+    uint32_t blkmask[kPrefetch];
+#define LAMB_FETCH(S)                                                              \
+    {                                                                              \
+      uint32_t bm = 0;                                                             \
+      int st = snext;                                                              \
+      while (st < nsteps) {                                                        \
+        const uint32_t g = uni(step_g[st]);                                        \
+        ++st;                                                                      \
+        const uint32_t rg = (g & 0xFFFFu) >> gsh_r, cg = (g >> 16) >> gsh_c;       \
+        bm = ((rg & 0xFu) && (cg & 0xFu) ? 1u : 0u) |                              \
+             ((rg & 0xFu) && (cg & 0xF0u) ? 2u : 0u) |                             \
+             ((rg & 0xF0u) && (cg & 0xFu) ? 4u : 0u) |                             \
+             ((rg & 0xF0u) && (cg & 0xF0u) ? 8u : 0u);                             \
+        if (bm) break;                                                             \
+      }                                                                            \
+      snext = st;                                                                  \
+      blkmask[S] = bm;                                                             \
+      if (bm) {                                                                    \
+        const int Q = 4 * (st - 1) + kc;                                           \
+        const uint32_t ca = col_a[Q], cb = col_b[Q], cm = col_m[Q];                \
+        _Pragma("unroll") for (int h = 0; h < 2; ++h) {                            \
+          double v = 0.0;                                                          \
+          if ((bm & (3u << (2 * h))) && ca != kNoCol && rpos[h] >= 0 &&            \
+              ((cm >> rpos[h]) & 1u))                                              \
+            v = a_pack[(int64_t)ca * 64 + 32 * wr + 16 * h + (lane & 15)];         \
+          ra[S][h] = v;                                                            \
+        }                                                                          \
+        _Pragma("unroll") for (int q = 0; q < 2; ++q) {                            \
+          double v = 0.0;                                                          \
+          if ((bm & (5u << q)) && ca != kNoCol && cpos[q] >= 0 &&                  \
+              ((cm >> (16 + cpos[q])) & 1u))                                       \
+            v = b_pack[(int64_t)cb * 64 + 32 * wc + 16 * q + (lane & 15)];         \
+          rb[S][q] = v;                                                            \
+        }                                                                          \
+      }                                                                            \
     }
-#define LAMB_MFMA(S)                                                            \
-    {                                                                           \
-      const uint32_t bm = blkmask[S];                                           \
-      _Pragma("unroll") for (int h = 0; h < 2; ++h)                             \
-        _Pragma("unroll") for (int q = 0; q < 2; ++q)                           \
-          if ((bm >> (2 * h + q)) & 1u)                                         \
+#define LAMB_MFMA(S)                                                               \
+    {                                                                              \
+      const uint32_t bm = blkmask[S];                                              \
+      _Pragma("unroll") for (int h = 0; h < 2; ++h)                                \
+        _Pragma("unroll") for (int q = 0; q < 2; ++q)                              \
+          if ((bm >> (2 * h + q)) & 1u)                                            \
             acc[2 * h + q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[S][h], rb[S][q], acc[2 * h + q], 0, 0, 0); \
     }
-    // prologue: fill the ring
-    _Pragma("unroll") for (int S = 0; S < kPrefetch; ++S) {
-      if (more) {
-        LAMB_FETCH(S);
-      } else {
-        blkmask[S] = 0;
-      }
-    }
-    // steady state: consume slot S, refill it kPrefetch steps ahead
-    bool live = true;
-    while (live) {
-      live = false;
+    _Pragma("unroll") for (int S = 0; S < kPrefetch; ++S) LAMB_FETCH(S);
+    while (blkmask[0] | blkmask[1] | blkmask[2] | blkmask[3]) {
       _Pragma("unroll") for (int S = 0; S < kPrefetch; ++S) {
-        if (blkmask[S]) live = true;
         LAMB_MFMA(S);
-        if (more) {
-          LAMB_FETCH(S);
-          live = true;
-        } else {
-          blkmask[S] = 0;
-        }
+        LAMB_FETCH(S);
       }
     }
 #undef LAMB_FETCH
