@@ -82,7 +82,8 @@ def main():
 
     for w in range(args.warmup):
         st = D.multiply(da, db, dc, eps_filter=eps)
-        print(f"warmup {w}: {st['ms_total']:.1f} ms (numeric {st['ms_numeric']:.2f} ms)",
+        print(f"warmup {w}: {st['ms_total']:.1f} ms (strips+tiles {st['ms_symbolic']:.2f} ms, "
+              f"numeric {st['ms_numeric']:.2f} ms, C index {st['ms_index']:.2f} ms)",
               file=sys.stderr, flush=True)
     sts = []
     t0 = time.perf_counter()
@@ -123,6 +124,8 @@ def main():
             "hbm_gb_per_s_step": bytes_alg / (ms_step * 1e-3) / 1e9,
             "hbm_gb_per_s_numeric": bytes_alg / (ms_num * 1e-3) / 1e9,
             "ms_numeric": ms_num,
+            "ms_symbolic": float(np.mean([s["ms_symbolic"] for s in sts])),
+            "ms_c_index": float(np.mean([s["ms_index"] for s in sts])),
             "tiles": st["n_tiles"],
             "tile_entries": st["n_tile_entries"],
         },
@@ -138,7 +141,13 @@ def main():
     }
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(a, b, eps, args.cpu_rows)
-    for h in (dc, db, da, dd):
+    # Calibration for hardware byte counters: one device copy of A's data
+    # pool (a plain read + write of a known byte count) in the same process.
+    dcopy = D.DeviceMatrix(dd)
+    dcopy.copy_from(da)
+    print(f"copy calibration: {8 * a.data.size} bytes read and written", file=sys.stderr,
+          flush=True)
+    for h in (dcopy, dc, db, da, dd):
         h.close()
     ctx.close()
     print(json.dumps(result), flush=True)

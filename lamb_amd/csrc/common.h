@@ -79,7 +79,7 @@ hipError_t launch_block_nsq(hipStream_t s, int64_t nblk, const int32_t* d_rows,
                             const int32_t* d_cols, const int32_t* d_blk, const int64_t* d_off,
                             const double* d_data, double* d_nsq);
 hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* tiles,
-                              const Entry* entries, const Band* bands,
+                              const int32_t* perm, const Entry* entries, const Band* bands,
                               const double* a_pack, const double* b_pack, double alpha,
                               double beta, double eps2, const int64_t* old_row_p,
                               const int32_t* old_col, const int64_t* old_off,

@@ -121,12 +121,13 @@ __device__ __forceinline__ double wave_sum(double v) {
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
 __global__ __launch_bounds__(256, 4) void numeric_f64_kernel(
-    const TileDesc* __restrict__ tiles, const Entry* __restrict__ entries,
-    const Band* __restrict__ bands, const double* __restrict__ a_pack,
-    const double* __restrict__ b_pack, double alpha, double beta, double eps2,
-    const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
-    const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
-    double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
+    const TileDesc* __restrict__ tiles, const int32_t* __restrict__ perm,
+    const Entry* __restrict__ entries, const Band* __restrict__ bands,
+    const double* __restrict__ a_pack, const double* __restrict__ b_pack, double alpha,
+    double beta, double eps2, const int64_t* __restrict__ old_row_p,
+    const int32_t* __restrict__ old_col, const int64_t* __restrict__ old_off,
+    const double* __restrict__ old_data, double* __restrict__ c_data,
+    uint16_t* __restrict__ kept_out) {
   // main loop: the step table of a window of K columns; epilogue: C tile
   __shared__ double lds[kBandRows * kPitch];
   uint32_t* col_a = reinterpret_cast<uint32_t*>(lds);  // packed A panel column
@@ -153,7 +154,7 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_kernel(
   const int nb = gridDim.x;
   const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
   const int per = nb >> 3, rem = nb & 7;
-  const int tile = (xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per) + slot;
+  const int tile = perm[(xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per) + slot];
   const TileDesc* tp = tiles + tile;
   const int I = tp->I, J = tp->J, e0 = tp->e0, ne = tp->ne, ktot = tp->ktot;
   const int64_t out_base = tp->out_base;
@@ -375,13 +376,13 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_kernel(
 }
 
 hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* tiles,
-                              const Entry* entries, const Band* bands,
+                              const int32_t* perm, const Entry* entries, const Band* bands,
                               const double* a_pack, const double* b_pack, double alpha,
                               double beta, double eps2, const int64_t* old_row_p,
                               const int32_t* old_col, const int64_t* old_off,
                               const double* old_data, double* c_data, uint16_t* kept) {
   if (n_tiles == 0) return hipSuccess;
-  hipLaunchKernelGGL(numeric_f64_kernel, dim3((unsigned)n_tiles), dim3(256), 0, s, tiles,
+  hipLaunchKernelGGL(numeric_f64_kernel, dim3((unsigned)n_tiles), dim3(256), 0, s, tiles, perm,
                      entries, bands, a_pack, b_pack, alpha, beta, eps2, old_row_p, old_col,
                      old_off, old_data, c_data, kept);
   return hipGetLastError();

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <array>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -174,6 +175,12 @@ extern "C" int32_t lamb_hip_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A_,
   if (S.r.n_entries > INT32_MAX || S.r.n_tiles > INT32_MAX)
     return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_OVERFLOW_ERROR,
                      "multiply: symbolic sizes exceed 32-bit indices");
+  {
+    // tiles run in blocks of rb row bands (tuning knob, default 16)
+    const char* env = getenv("LAMB_HIP_TILE_ROWBLOCK");
+    const int rb = env ? atoi(env) : 16;
+    HIPCHK(ctx, build_tile_order(s, S.W, D, S.r, std::max(1, std::min(rb, 32))), "tile order");
+  }
   HIPCHK(ctx, hipEventRecord(ctx->ev[3], s), "event");
 
   // ---- numeric ----
@@ -185,6 +192,7 @@ extern "C" int32_t lamb_hip_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A_,
          "C pool");
   HIPCHK(ctx, hipEventRecord(ctx->ev[0], s), "event");
   HIPCHK(ctx, launch_numeric_f64(s, n_tiles, (const TileDesc*)S.r.tiles.p,
+                                 (const int32_t*)S.r.perm.p,
                                  (const Entry*)S.r.entries.p, D.bands,
                                  (const double*)ctx->d_apack.p, (const double*)ctx->d_bpack.p,
                                  alpha, beta, eps2, use_old ? MC.row_p : nullptr,

@@ -46,10 +46,10 @@ struct StripSet {
 
 struct SymScratch {
   DevBuf merged, flag, excl, scan, gslot;
-  DevBuf cnt, ksum, pat, stats, band4, band4s;
+  DevBuf cnt, ksum, pat, stats, band4, band4s, cells, cellcnt, cellbase;
   void release() {
     for (DevBuf* b : {&merged, &flag, &excl, &scan, &gslot, &cnt, &ksum, &pat, &stats, &band4,
-                      &band4s})
+                      &band4s, &cells, &cellcnt, &cellbase})
       b->release();
   }
 };
@@ -58,8 +58,9 @@ struct SymResult {
   int64_t n_tiles = 0, n_entries = 0, c_size = 0, n_struct = 0;
   int64_t kept = 0, dropped = 0;
   double flops = 0.0;
-  DevBuf tiles, entries, tile_base;
+  DevBuf tiles, entries, tile_base, perm;
   void release() {
+    perm.release();
     tiles.release();
     entries.release();
     tile_base.release();
@@ -75,6 +76,10 @@ hipError_t build_strips(hipStream_t s, SymScratch& W, const MatView& M, const Di
                         bool a_side, StripSet& out);
 hipError_t build_tiles(hipStream_t s, SymScratch& W, const DistView& D, const StripSet& A,
                        const StripSet& B, const MatView* Cold, double eps2, SymResult& r);
+// perm[k] = tile run k-th by the numeric kernel (blocks of rb row bands,
+// column band by column band)
+hipError_t build_tile_order(hipStream_t s, SymScratch& W, const DistView& D, SymResult& r,
+                            int rb);
 hipError_t build_c_index(hipStream_t s, SymScratch& W, const DistView& D, const SymResult& r,
                          const uint16_t* kept, CIndex& c);
 

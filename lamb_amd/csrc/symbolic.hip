@@ -501,6 +501,37 @@ __global__ __launch_bounds__(64) void cidx_fill(int32_t nbands, const Band* __re
 }
 
 // ---------------------------------------------------------------------------
+// Tile schedule. Tiles are stored band-major (I, J). The numeric kernel walks
+// them in blocks of rb row bands, column band by column band: order key
+// (I / rb, J, I % rb), so the rb tiles that share the B strips of column
+// band J run back to back. perm[position] = tile.
+// ---------------------------------------------------------------------------
+__global__ void tile_cells(int64_t ntiles, int rb, int32_t nbands, const TileDesc* __restrict__ tiles,
+                           uint32_t* __restrict__ cell_mask) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  const int32_t I = tiles[t].I, J = tiles[t].J;
+  atomicOr(&cell_mask[(int64_t)(I / rb) * nbands + J], 1u << (I % rb));
+}
+
+__global__ void cell_counts(int64_t ncells, const uint32_t* __restrict__ cell_mask,
+                            int64_t* __restrict__ cnt) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < ncells) cnt[c] = __builtin_popcount(cell_mask[c]);
+}
+
+__global__ void tile_perm(int64_t ntiles, int rb, int32_t nbands, const TileDesc* __restrict__ tiles,
+                          const uint32_t* __restrict__ cell_mask, const int64_t* __restrict__ base,
+                          int32_t* __restrict__ perm) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  const int32_t I = tiles[t].I, J = tiles[t].J;
+  const int64_t c = (int64_t)(I / rb) * nbands + J;
+  const uint32_t below = cell_mask[c] & ((1u << (I % rb)) - 1u);
+  perm[base[c] + __builtin_popcount(below)] = (int32_t)t;
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 #define CK(expr)                             \
@@ -671,6 +702,32 @@ hipError_t build_c_index(hipStream_t s, SymScratch& W, const DistView& D, const 
   CK(hipStreamSynchronize(s));
   c.nelems = (int64_t)ne;
   return hipSuccess;
+}
+
+
+hipError_t build_tile_order(hipStream_t s, SymScratch& W, const DistView& D, SymResult& r,
+                            int rb) {
+  const int32_t nb = D.nbands;
+  const int64_t nt = r.n_tiles;
+  CK(r.perm.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(nt, 1)));
+  if (nt == 0) return hipSuccess;
+  const int64_t ncells = (int64_t)((nb + rb - 1) / rb) * nb;
+  CK(W.cells.ensure(sizeof(uint32_t) * (size_t)ncells));
+  CK(W.cellcnt.ensure(sizeof(int64_t) * (size_t)(ncells + 1)));
+  CK(W.cellbase.ensure(sizeof(int64_t) * (size_t)(ncells + 1)));
+  CK(W.scan.ensure(sizeof(int64_t) * (size_t)scan_scratch_elems(ncells + 1)));
+  CK(fill32(s, (int32_t*)W.cells.p, ncells, 0));
+  hipLaunchKernelGGL(tile_cells, dim3(grid_of(nt)), dim3(256), 0, s, nt, rb, nb,
+                     (const TileDesc*)r.tiles.p, (uint32_t*)W.cells.p);
+  hipLaunchKernelGGL(cell_counts, dim3(grid_of(ncells)), dim3(256), 0, s, ncells,
+                     (const uint32_t*)W.cells.p, (int64_t*)W.cellcnt.p);
+  CK(hipGetLastError());
+  CK(exclusive_scan(s, (const int64_t*)W.cellcnt.p, (int64_t*)W.cellbase.p, ncells,
+                    (int64_t*)W.scan.p));
+  hipLaunchKernelGGL(tile_perm, dim3(grid_of(nt)), dim3(256), 0, s, nt, rb, nb,
+                     (const TileDesc*)r.tiles.p, (const uint32_t*)W.cells.p,
+                     (const int64_t*)W.cellbase.p, (int32_t*)r.perm.p);
+  return hipGetLastError();
 }
 
 }  // namespace lamb
