@@ -19,6 +19,7 @@ FLAGS = [
     # packed FP32 VALU forms are kept out of every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result",
+    *os.environ.get("LAMB_HIP_EXTRA_FLAGS", "").split(),
 ]
 
 

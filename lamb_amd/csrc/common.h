@@ -80,7 +80,8 @@ hipError_t launch_block_nsq(hipStream_t s, int64_t nblk, const int32_t* d_rows,
                             const double* d_data, double* d_nsq);
 hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* tiles,
                               const int32_t* perm, const Entry* entries, const Band* bands,
-                              const double* a_pack, const double* b_pack, double alpha,
+                              const double* a_pack, const double* b_pack, uint32_t a_zero,
+                              uint32_t b_zero, double alpha,
                               double beta, double eps2, const int64_t* old_row_p,
                               const int32_t* old_col, const int64_t* old_off,
                               const double* old_data, double* c_data, uint16_t* kept);

@@ -36,6 +36,7 @@ hipError_t launch_block_nsq(hipStream_t s, int64_t nblk, const int32_t* d_rows,
 // ---------------------------------------------------------------------------
 // Strip packing: each strip (the blocks of one band against one atom k)
 // becomes a dense [kdim][64] panel, zero where a band atom has no block.
+// pk_off[n_strips] (the panel total) is followed by one zero column.
 // A strips (row_side): panel[kk][r] = A_{i(r),k}(r_in, kk).
 // B strips:            panel[kk][c] = B_{k,j(c)}(kk, c_in).
 // One wave per strip; lane = band row / column.
@@ -48,6 +49,9 @@ __global__ __launch_bounds__(256) void pack_strips_kernel(int64_t n_strips,
                                                           const double* __restrict__ data,
                                                           int row_side, double* __restrict__ packed) {
   const int lane = threadIdx.x & 63;
+  // the panel column after the last strip is all zeros (the numeric kernel
+  // points masked-off operand lanes at it)
+  if (blockIdx.x == 0 && threadIdx.x < 64) packed[pk_off[n_strips] + lane] = 0.0;
   const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (s >= n_strips) return;
   const int32_t band = strips[s].band;
@@ -92,16 +96,18 @@ hipError_t launch_pack_strips(hipStream_t s, int64_t n_strips, const Strip* stri
 // 64 x 64 values; wave (wr, wc) owns the 32 x 32 quadrant rows 32wr..,
 // cols 32wc.. as four 16 x 16 v_mfma_f64_16x16x4 accumulators.
 //
-// The tile's entries (strip pairs, one K atom each) are staged in LDS once;
-// then every wave streams, on its own and without barriers, the K columns
-// of the entries that touch its quadrant (an entry touches it when one of
-// its row atoms overlaps the wave's 32 rows and one of its column atoms
-// overlaps the wave's 32 columns). Four such columns form a K step. MFMA
-// operands are loaded straight from the packed [k][64] panels into a ring
-// of register slots, kPrefetch steps ahead of the MFMAs that consume them.
-// A 16 x 16 block of a step is skipped (no loads, no MFMA) when its 16 rows
-// or its 16 columns hold no atom of the step's entries; single rows /
-// columns of absent atoms are not loaded (their operand is zero).
+// The tile's entries (strip pairs, one K atom each) are staged in LDS as a
+// table of K columns (A panel column, B panel column, row / column atom
+// masks); four consecutive columns form a K step. Every wave then walks, on
+// its own and without barriers, the steps that touch its quadrant: 64 steps
+// at a time, one lane per step computes the step's 4-bit mask of live 16 x 16
+// blocks, a ballot collects the live steps, and the wave takes them in order
+// (s_ff1 + readlane). MFMA operands are loaded straight from the packed
+// [k][64] panels into a ring of register slots, kPrefetch steps ahead of the
+// MFMAs that consume them. Every step issues exactly four loads: a lane whose
+// operand is zero (atom outside the entry's masks, block not live, step past
+// the end) loads from the zero column after the panels instead, so the load
+// count is static and the MFMAs wait only for their own slot.
 //
 // v_mfma_f64_16x16x4 lane layout (lane l): A[i][k] at l = i + 16k,
 // B[k][j] at l = j + 16k, D[i][j] in lane 16*(i%4) + j, register i/4.
@@ -110,8 +116,13 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 // K columns of the per-tile step table held in LDS at a time
 constexpr int kTableCols = 2048;
-constexpr int kPrefetch = 4;  // K steps in flight per wave
-constexpr uint32_t kNoCol = 0xFFFFFFFFu;
+#ifndef LAMB_NUMERIC_PREFETCH
+#define LAMB_NUMERIC_PREFETCH 4
+#endif
+#ifndef LAMB_NUMERIC_WAVES
+#define LAMB_NUMERIC_WAVES 4
+#endif
+constexpr int kPrefetch = LAMB_NUMERIC_PREFETCH;  // K steps in flight per wave
 
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -120,20 +131,29 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
-__global__ __launch_bounds__(256, 4) void numeric_f64_kernel(
+// 4-bit mask of the live 16 x 16 blocks of a wave's 32 x 32 quadrant in a
+// step whose row / column 4-groups are g (rows low half, columns high half):
+// bit 2h+q = row half h and column half q both hold a live group.
+__device__ __forceinline__ uint32_t quadrant_blocks(uint32_t g, uint32_t gsh_r, uint32_t gsh_c) {
+  const uint32_t rg = (g & 0xFFFFu) >> gsh_r, cg = (g >> 16) >> gsh_c;
+  const uint32_t r0 = (rg & 0xFu) ? 1u : 0u, r1 = (rg & 0xF0u) ? 1u : 0u;
+  const uint32_t c0 = (cg & 0xFu) ? 1u : 0u, c1 = (cg & 0xF0u) ? 1u : 0u;
+  return (r0 & c0) | ((r0 & c1) << 1) | ((r1 & c0) << 2) | ((r1 & c1) << 3);
+}
+
+__global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
     const TileDesc* __restrict__ tiles, const int32_t* __restrict__ perm,
     const Entry* __restrict__ entries, const Band* __restrict__ bands,
-    const double* __restrict__ a_pack, const double* __restrict__ b_pack, double alpha,
-    double beta, double eps2, const int64_t* __restrict__ old_row_p,
-    const int32_t* __restrict__ old_col, const int64_t* __restrict__ old_off,
-    const double* __restrict__ old_data, double* __restrict__ c_data,
-    uint16_t* __restrict__ kept_out) {
+    const double* __restrict__ a_pack, const double* __restrict__ b_pack, uint32_t a_zero,
+    uint32_t b_zero, double alpha, double beta, double eps2,
+    const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
+    const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
+    double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
   // main loop: the step table of a window of K columns; epilogue: C tile
   __shared__ double lds[kBandRows * kPitch];
-  uint32_t* col_a = reinterpret_cast<uint32_t*>(lds);  // packed A panel column
-  uint32_t* col_b = col_a + kTableCols;                // packed B panel column
-  uint32_t* col_m = col_b + kTableCols;                // rmask | cmask << 16
-  uint32_t* step_g = col_m + kTableCols;               // row groups | col groups << 16
+  uint2* col_ab = reinterpret_cast<uint2*>(lds);  // A, B panel column
+  uint32_t* col_m = reinterpret_cast<uint32_t*>(col_ab + kTableCols);  // rmask | cmask << 16
+  uint32_t* step_g = col_m + kTableCols;  // row groups | col groups << 16
   static_assert(sizeof(uint32_t) * (3 * kTableCols + kTableCols / 4) <=
                     sizeof(double) * kBandRows * kPitch, "step table exceeds LDS buffer");
   __shared__ int16_t rs_r[kBandAtoms + 1], rs_c[kBandAtoms + 1];
@@ -150,7 +170,7 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_kernel(
   const int wr = wave >> 1, wc = wave & 1;
 
   // Workgroups are dealt round-robin over the 8 XCDs; remap so that each XCD
-  // walks one contiguous range of tiles (shared A strips stay in its L2).
+  // walks one contiguous range of the tile order.
   const int nb = gridDim.x;
   const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
   const int per = nb >> 3, rem = nb & 7;
@@ -177,24 +197,27 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_kernel(
       for (int q = rs_c[tid] / 4; q <= (rs_c[tid + 1] - 1) / 4; ++q) g |= 1u << q;
     pg_c[tid] = (uint16_t)g;
   }
-  // band position of this lane's operand rows (A) and columns (B)
-  int rpos[2] = {-1, -1}, cpos[2] = {-1, -1};
+  // mask bit of the band atom holding this lane's operand rows (A) and
+  // columns (B); 0 for rows / columns past the band
+  uint32_t rbit[2] = {0u, 0u}, cbit[2] = {0u, 0u};
   for (int a = 0; a < natI; ++a)
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int r = 32 * wr + 16 * h + (lane & 15);
-      if (r >= rs_r[a] && r < rs_r[a + 1]) rpos[h] = a;
+      if (r >= rs_r[a] && r < rs_r[a + 1]) rbit[h] = 1u << a;
     }
   for (int a = 0; a < natJ; ++a)
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const int c = 32 * wc + 16 * q + (lane & 15);
-      if (c >= rs_c[a] && c < rs_c[a + 1]) cpos[q] = a;
+      if (c >= rs_c[a] && c < rs_c[a + 1]) cbit[q] = 1u << (16 + a);
     }
   const int kc = lane >> 4;  // K column of this lane within a step
   // the 16-bit row / column group masks (4 rows or columns per group) of
   // this wave's halves and quads: rows 32wr+16h.., cols 32wc+16q..
   const uint32_t gsh_r = 8 * wr, gsh_c = 8 * wc;
+  const double* a_lane = a_pack + 32 * wr + (lane & 15);
+  const double* b_lane = b_pack + 32 * wc + (lane & 15);
 
   dbl4 acc[4];
 #pragma unroll
@@ -204,7 +227,10 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_kernel(
     const int ncols = min(kTableCols, ktot - Q0);
     const int nsteps = (ncols + 3) >> 2;
     __syncthreads();  // previous window done (and pg_r / pg_c visible)
-    for (int t = tid; t < ((ncols + 3) & ~3); t += 256) col_a[t] = kNoCol;
+    for (int t = tid; t < 4 * nsteps; t += 256) {
+      col_ab[t] = make_uint2(a_zero, b_zero);
+      col_m[t] = 0u;
+    }
     for (int t = tid; t < nsteps; t += 256) step_g[t] = 0;
     __syncthreads();
     // one thread per entry writes its columns that fall in this window
@@ -223,51 +249,48 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_kernel(
       const uint32_t g = gr | (gc << 16);
       for (int Q = lo; Q < hi; ++Q) {
         const int q = Q - Q0;
-        col_a[q] = en.a_col + (uint32_t)(Q - ks);
-        col_b[q] = en.b_col + (uint32_t)(Q - ks);
+        col_ab[q] = make_uint2(en.a_col + (uint32_t)(Q - ks), en.b_col + (uint32_t)(Q - ks));
         col_m[q] = m;
       }
       for (int st = (lo - Q0) >> 2; st <= (hi - 1 - Q0) >> 2; ++st) atomicOr(&step_g[st], g);
     }
     __syncthreads();
 
-    // ---- per-wave walk over the window's steps, kPrefetch steps ahead ----
-    int snext = 0;  // next step to examine
-    double ra[kPrefetch][2], rb[kPrefetch][2];
-    uint32_t blkmask[kPrefetch];
+    // ---- per-wave walk over the window's live steps, kPrefetch ahead ----
+    int cbase = -64;        // first step of the current chunk
+    uint64_t live = 0;      // live steps of the chunk not yet taken
+    uint32_t chunk_bm = 0;  // lane l: block mask of step cbase + l
+    double ra[8][2], rb[8][2];  // slots 0..kPrefetch-1 used
+    uint32_t blkmask[8];
 #define LAMB_FETCH(S)                                                              \
     {                                                                              \
-      uint32_t bm = 0;                                                             \
-      int st = snext;                                                              \
-      while (st < nsteps) {                                                        \
-        const uint32_t g = uni(step_g[st]);                                        \
-        ++st;                                                                      \
-        const uint32_t rg = (g & 0xFFFFu) >> gsh_r, cg = (g >> 16) >> gsh_c;       \
-        bm = ((rg & 0xFu) && (cg & 0xFu) ? 1u : 0u) |                              \
-             ((rg & 0xFu) && (cg & 0xF0u) ? 2u : 0u) |                             \
-             ((rg & 0xF0u) && (cg & 0xFu) ? 4u : 0u) |                             \
-             ((rg & 0xF0u) && (cg & 0xF0u) ? 8u : 0u);                             \
-        if (bm) break;                                                             \
+      while (live == 0 && cbase + 64 < nsteps) {                                   \
+        cbase += 64;                                                               \
+        const int sl = cbase + lane;                                               \
+        chunk_bm = quadrant_blocks(sl < nsteps ? step_g[sl] : 0u, gsh_r, gsh_c);   \
+        live = __ballot(chunk_bm != 0u);                                           \
       }                                                                            \
-      snext = st;                                                                  \
+      uint32_t bm = 0;                                                             \
+      int st = 0;                                                                  \
+      if (live) {                                                                  \
+        const int l = __builtin_ctzll(live);                                       \
+        live &= live - 1;                                                          \
+        st = cbase + l;                                                            \
+        bm = __builtin_amdgcn_readlane(chunk_bm, l);                               \
+      }                                                                            \
       blkmask[S] = bm;                                                             \
-      if (bm) {                                                                    \
-        const int Q = 4 * (st - 1) + kc;                                           \
-        const uint32_t ca = col_a[Q], cb = col_b[Q], cm = col_m[Q];                \
-        _Pragma("unroll") for (int h = 0; h < 2; ++h) {                            \
-          double v = 0.0;                                                          \
-          if ((bm & (3u << (2 * h))) && ca != kNoCol && rpos[h] >= 0 &&            \
-              ((cm >> rpos[h]) & 1u))                                              \
-            v = a_pack[(int64_t)ca * 64 + 32 * wr + 16 * h + (lane & 15)];         \
-          ra[S][h] = v;                                                            \
-        }                                                                          \
-        _Pragma("unroll") for (int q = 0; q < 2; ++q) {                            \
-          double v = 0.0;                                                          \
-          if ((bm & (5u << q)) && ca != kNoCol && cpos[q] >= 0 &&                  \
-              ((cm >> (16 + cpos[q])) & 1u))                                       \
-            v = b_pack[(int64_t)cb * 64 + 32 * wc + 16 * q + (lane & 15)];         \
-          rb[S][q] = v;                                                            \
-        }                                                                          \
+      const int Q = 4 * st + kc;                                                   \
+      const uint2 ab = col_ab[Q];                                                  \
+      const uint32_t cm = col_m[Q];                                                \
+      _Pragma("unroll") for (int h = 0; h < 2; ++h) {                              \
+        const uint32_t need = (bm & (3u << (2 * h))) ? rbit[h] : 0u;               \
+        const uint32_t c = (cm & need) ? ab.x : a_zero;                            \
+        ra[S][h] = a_lane[(int64_t)c * 64 + 16 * h];                               \
+      }                                                                            \
+      _Pragma("unroll") for (int q = 0; q < 2; ++q) {                              \
+        const uint32_t need = (bm & (5u << q)) ? cbit[q] : 0u;                     \
+        const uint32_t c = (cm & need) ? ab.y : b_zero;                            \
+        rb[S][q] = b_lane[(int64_t)c * 64 + 16 * q];                               \
       }                                                                            \
     }
 #define LAMB_MFMA(S)                                                               \
@@ -278,13 +301,23 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_kernel(
           if ((bm >> (2 * h + q)) & 1u)                                            \
             acc[2 * h + q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[S][h], rb[S][q], acc[2 * h + q], 0, 0, 0); \
     }
-    _Pragma("unroll") for (int S = 0; S < kPrefetch; ++S) LAMB_FETCH(S);
-    while (blkmask[0] | blkmask[1] | blkmask[2] | blkmask[3]) {
-      _Pragma("unroll") for (int S = 0; S < kPrefetch; ++S) {
-        LAMB_MFMA(S);
-        LAMB_FETCH(S);
-      }
+    // slots spelled out so that every register array index is a constant
+#define LAMB_SLOTS(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
+    static_assert(kPrefetch >= 1 && kPrefetch <= 8, "prefetch depth 1..8");
+#define LAMB_PRIME(S) if (S < kPrefetch) LAMB_FETCH(S)
+#define LAMB_ANY(S) if (S < kPrefetch) any |= blkmask[S];
+#define LAMB_STEP(S) if (S < kPrefetch) { LAMB_MFMA(S); LAMB_FETCH(S); }
+    LAMB_SLOTS(LAMB_PRIME)
+    for (;;) {
+      uint32_t any = 0;
+      LAMB_SLOTS(LAMB_ANY)
+      if (!any) break;
+      LAMB_SLOTS(LAMB_STEP)
     }
+#undef LAMB_STEP
+#undef LAMB_ANY
+#undef LAMB_PRIME
+#undef LAMB_SLOTS
 #undef LAMB_FETCH
 #undef LAMB_MFMA
   }
@@ -377,13 +410,14 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_kernel(
 
 hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* tiles,
                               const int32_t* perm, const Entry* entries, const Band* bands,
-                              const double* a_pack, const double* b_pack, double alpha,
+                              const double* a_pack, const double* b_pack, uint32_t a_zero,
+                              uint32_t b_zero, double alpha,
                               double beta, double eps2, const int64_t* old_row_p,
                               const int32_t* old_col, const int64_t* old_off,
                               const double* old_data, double* c_data, uint16_t* kept) {
   if (n_tiles == 0) return hipSuccess;
   hipLaunchKernelGGL(numeric_f64_kernel, dim3((unsigned)n_tiles), dim3(256), 0, s, tiles, perm,
-                     entries, bands, a_pack, b_pack, alpha, beta, eps2, old_row_p, old_col,
+                     entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta, eps2, old_row_p, old_col,
                      old_off, old_data, c_data, kept);
   return hipGetLastError();
 }

@@ -154,13 +154,14 @@ extern "C" int32_t lamb_hip_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A_,
   // ---- strips and packed panels ----
   HIPCHK(ctx, build_strips(s, S.W, MA, D, true, S.sa), "A strips");
   HIPCHK(ctx, build_strips(s, S.W, MB, D, false, S.sb), "B strips");
-  if (S.sa.pk_total / 64 > (int64_t)UINT32_MAX || S.sb.pk_total / 64 > (int64_t)UINT32_MAX)
+  // panel columns are addressed by 32-bit indices; one zero column follows
+  // the last panel of each side
+  if (S.sa.pk_total / 64 >= (int64_t)UINT32_MAX || S.sb.pk_total / 64 >= (int64_t)UINT32_MAX)
     return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_OVERFLOW_ERROR,
                      "multiply: packed panels exceed 32-bit column indices");
-  HIPCHK(ctx, ctx->d_apack.ensure(sizeof(double) * (size_t)std::max<int64_t>(S.sa.pk_total, 1)),
-         "a pack alloc");
-  HIPCHK(ctx, ctx->d_bpack.ensure(sizeof(double) * (size_t)std::max<int64_t>(S.sb.pk_total, 1)),
-         "b pack alloc");
+  const uint32_t a_zero = (uint32_t)(S.sa.pk_total / 64), b_zero = (uint32_t)(S.sb.pk_total / 64);
+  HIPCHK(ctx, ctx->d_apack.ensure(sizeof(double) * (size_t)(S.sa.pk_total + 64)), "a pack alloc");
+  HIPCHK(ctx, ctx->d_bpack.ensure(sizeof(double) * (size_t)(S.sb.pk_total + 64)), "b pack alloc");
   HIPCHK(ctx, launch_pack_strips(s, S.sa.n, (const Strip*)S.sa.strips.p,
                                  (const int64_t*)S.sa.pk.p, D.bands, D.blk,
                                  (const double*)A->d_data.p, true, (double*)ctx->d_apack.p),
@@ -195,7 +196,7 @@ extern "C" int32_t lamb_hip_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A_,
                                  (const int32_t*)S.r.perm.p,
                                  (const Entry*)S.r.entries.p, D.bands,
                                  (const double*)ctx->d_apack.p, (const double*)ctx->d_bpack.p,
-                                 alpha, beta, eps2, use_old ? MC.row_p : nullptr,
+                                 a_zero, b_zero, alpha, beta, eps2, use_old ? MC.row_p : nullptr,
                                  use_old ? MC.col : nullptr, use_old ? MC.off : nullptr,
                                  use_old ? (const double*)C->d_data.p : nullptr,
                                  (double*)newdata.p, (uint16_t*)ctx->d_kept.p),
