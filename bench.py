@@ -19,6 +19,10 @@ import numpy as np
 METRIC = "block-SpGEMM GFLOP/s & HBM GB/s: FP64 plain vs BF16x3 batched at ≤1e-9 err"
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per numeric_f64_kernel launch on this workload: rocprofv3
+# FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, the factors read off a device copy
+# of known size in the same run (profiles/numeric_f64_kernel_water6144.md).
+PROFILED_TRAFFIC_BYTES = 73.89e9
 
 
 def parse():
@@ -28,7 +32,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--seed", type=int, default=2)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    p.add_argument("--cpu-rows", type=int, default=4,
+    p.add_argument("--cpu-rows", type=int, default=2,
                    help="CPU baseline samples every n-th block row of A")
     return p.parse_args()
 
@@ -74,11 +78,16 @@ def main():
     lb = LambMatrix.from_global(dist, b)
     ctx = D.Context(0)
     dd = D.DeviceDistribution(ctx, dist)
+    # host sectors -> device pools (the boundary's upload and commit), timed
+    # to report the PCIe-inclusive rate beside the device-resident one
+    t_up = time.perf_counter()
     da = D.DeviceMatrix.from_lamb(dd, la)
     db = D.DeviceMatrix.from_lamb(dd, lb)
+    ms_upload = (time.perf_counter() - t_up) * 1e3
     dc = D.DeviceMatrix(dd)
     print(f"setup {time.perf_counter() - t_setup:.1f}s: {atoms.n_atoms} atoms, "
-          f"A {a.n_blocks} blocks, B {b.n_blocks} blocks", file=sys.stderr, flush=True)
+          f"A {a.n_blocks} blocks, B {b.n_blocks} blocks, upload+commit of A and B "
+          f"{ms_upload:.1f} ms", file=sys.stderr, flush=True)
 
     for w in range(args.warmup):
         st = D.multiply(da, db, dc, eps_filter=eps)
@@ -128,6 +137,8 @@ def main():
             "ms_c_index": float(np.mean([s["ms_index"] for s in sts])),
             "tiles": st["n_tiles"],
             "tile_entries": st["n_tile_entries"],
+            "ms_upload_ab": ms_upload,
+            "gflops_incl_upload": flops / ((ms_step + ms_upload) * 1e-3) / 1e9,
         },
         "roofline": {
             "bound": "mfma",
@@ -136,7 +147,8 @@ def main():
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tf / PEAK_FP64_TFLOPS,
-            "traffic": None,
+            "traffic": PROFILED_TRAFFIC_BYTES,
+            "algorithmic_bytes": bytes_alg,
         },
     }
     if not args.no_cpu:
