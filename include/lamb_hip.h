@@ -57,9 +57,10 @@ typedef struct lamb_hip_mat lamb_hip_mat;
 #define LAMB_HIP_UNIMPLEMENTED (-1000)
 #define LAMB_HIP_EXTERNAL_ERROR (-2000)
 
-/* multiply flags */
+/* multiply flags: mode in bits 0-7, BF16 slice count in bits 8-11 */
 #define LAMB_HIP_FP64 0          /* plain FP64 block products */
-#define LAMB_HIP_BF16_SPLIT 1    /* exponent-aligned BF16 slices, FP32 MFMA */
+#define LAMB_HIP_BF16_SPLIT 1    /* exponent-aligned BF16 slices, FP32 MFMA, FP64 across chains */
+#define LAMB_HIP_SLICES(n) ((n) << 8) /* with BF16_SPLIT: 3..5 slices (0 = default 5) */
 
 /* Statistics of one multiply (filled when a non-NULL pointer is given). */
 typedef struct lamb_hip_stats {

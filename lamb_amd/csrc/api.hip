@@ -103,6 +103,10 @@ int32_t lamb_hip_finalize(lamb_hip_ctx* ctx) {
   ctx->d_kept.release();
   ctx->d_apack.release();
   ctx->d_bpack.release();
+  ctx->d_aexp.release();
+  ctx->d_bexp.release();
+  ctx->d_aslice.release();
+  ctx->d_bslice.release();
   for (auto& ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -210,6 +214,11 @@ int32_t lamb_hip_dist_create(lamb_hip_ctx* ctx, int64_t n_atoms, const int8_t* b
          "dist band_of");
   HIPCHK(ctx, d->d_pos_in_band.upload(ctx->stream, d->pos_in_band.data(), sizeof(int32_t) * d->n),
          "dist pos_in_band");
+  std::vector<int64_t> elem0(d->n + 1, 0);  // alive until the stream sync below
+  for (int32_t p = 0; p < d->n; ++p) elem0[p + 1] = elem0[p] + d->blk_int[p];
+  d->n_elem = elem0[d->n];
+  HIPCHK(ctx, d->d_elem0.upload(ctx->stream, elem0.data(), sizeof(int64_t) * (d->n + 1)),
+         "dist elem0");
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "dist sync");
   *out = d.release();
   return LAMB_HIP_OK;
@@ -222,6 +231,7 @@ int32_t lamb_hip_dist_destroy(lamb_hip_dist* d) {
   d->d_bands.release();
   d->d_band_of.release();
   d->d_pos_in_band.release();
+  d->d_elem0.release();
   delete d;
   return LAMB_HIP_OK;
 }

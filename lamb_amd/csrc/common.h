@@ -85,6 +85,23 @@ hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* ti
                               double beta, double eps2, const int64_t* old_row_p,
                               const int32_t* old_col, const int64_t* old_off,
                               const double* old_data, double* c_data, uint16_t* kept);
+// BF16-split numeric phase (bf16split.hip). ex: int32 per matrix row (A,
+// row_side) or column (B); slice packs: slices x (n_chunks + 1) x 64 x 16 B.
+hipError_t launch_slice_exponents(hipStream_t s, int64_t nblk, const int32_t* rows,
+                                  const int32_t* cols, const int32_t* blk, const int64_t* elem0,
+                                  const int64_t* off, const double* data, bool row_side,
+                                  int64_t n_elem, int32_t* ex);
+hipError_t launch_pack_slices(hipStream_t s, int slices, int64_t n_strips, const Strip* strips,
+                              const int64_t* pk_off, const Band* bands, const int32_t* blk,
+                              const int64_t* elem0, const int32_t* ex, const double* data,
+                              bool row_side, int64_t n_chunks, void* out);
+hipError_t launch_numeric_bf16(hipStream_t s, int slices, int32_t n_tiles, const TileDesc* tiles,
+                               const int32_t* perm, const Entry* entries, const Band* bands,
+                               const void* a_sl, const void* b_sl, uint32_t a_zero,
+                               uint32_t b_zero, double alpha, double beta, double eps2,
+                               const int64_t* old_row_p, const int32_t* old_col,
+                               const int64_t* old_off, const double* old_data, double* c_data,
+                               uint16_t* kept);
 hipError_t launch_trace(hipStream_t s, int32_t n_atoms, const int64_t* row_p,
                         const int32_t* col, const int64_t* off, const int32_t* blk,
                         const double* data, double* out);

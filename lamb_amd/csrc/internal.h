@@ -55,6 +55,8 @@ struct lamb_hip_ctx {
   std::string msg;
   // scratch reused across multiplies (symbolic.h)
   lamb::DevBuf d_kept, d_apack, d_bpack;
+  lamb::DevBuf d_aexp, d_bexp;  // BF16 split: exponent per A row / B column
+  lamb::DevBuf d_aslice, d_bslice;  // BF16 split: slice packs
   void* sym = nullptr;  // lamb::SymState*
 };
 
@@ -76,6 +78,8 @@ struct lamb_hip_dist {
   lamb::DevBuf d_blk;    // int32 by internal atom
   lamb::DevBuf d_bands;  // Band[]
   lamb::DevBuf d_band_of, d_pos_in_band;  // int32 by internal atom
+  lamb::DevBuf d_elem0;  // int64 [n+1]: first matrix row of each internal atom
+  int64_t n_elem = 0;    // matrix dimension (sum of block sizes)
 };
 
 struct lamb_hip_mat {

@@ -1,5 +1,6 @@
 // HIP kernels for gfx950: block norms, FP64 numeric phase, trace.
 #include "common.h"
+#include "tile_common.h"
 
 namespace lamb {
 
@@ -124,23 +125,6 @@ constexpr int kTableCols = 2048;
 #endif
 constexpr int kPrefetch = LAMB_NUMERIC_PREFETCH;  // K steps in flight per wave
 
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
-
-// 4-bit mask of the live 16 x 16 blocks of a wave's 32 x 32 quadrant in a
-// step whose row / column 4-groups are g (rows low half, columns high half):
-// bit 2h+q = row half h and column half q both hold a live group.
-__device__ __forceinline__ uint32_t quadrant_blocks(uint32_t g, uint32_t gsh_r, uint32_t gsh_c) {
-  const uint32_t rg = (g & 0xFFFFu) >> gsh_r, cg = (g >> 16) >> gsh_c;
-  const uint32_t r0 = (rg & 0xFu) ? 1u : 0u, r1 = (rg & 0xF0u) ? 1u : 0u;
-  const uint32_t c0 = (cg & 0xFu) ? 1u : 0u, c1 = (cg & 0xF0u) ? 1u : 0u;
-  return (r0 & c0) | ((r0 & c1) << 1) | ((r1 & c0) << 2) | ((r1 & c1) << 3);
-}
-
 __global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
     const TileDesc* __restrict__ tiles, const int32_t* __restrict__ perm,
     const Entry* __restrict__ entries, const Band* __restrict__ bands,
@@ -156,62 +140,16 @@ __global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
   uint32_t* step_g = col_m + kTableCols;  // row groups | col groups << 16
   static_assert(sizeof(uint32_t) * (3 * kTableCols + kTableCols / 4) <=
                     sizeof(double) * kBandRows * kPitch, "step table exceeds LDS buffer");
-  __shared__ int16_t rs_r[kBandAtoms + 1], rs_c[kBandAtoms + 1];
-  __shared__ uint16_t pg_r[kBandAtoms], pg_c[kBandAtoms];
-  __shared__ int32_t pair_id[kBandAtoms * kBandAtoms];
-  __shared__ int64_t pair_off[kBandAtoms * kBandAtoms];
-  __shared__ int32_t wave_cnt[4];
-  __shared__ int64_t wave_sz[4];
-  __shared__ uint32_t keptw[kBandAtoms];
+  __shared__ TileSmem sm;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wr = wave >> 1, wc = wave & 1;
-
-  // Workgroups are dealt round-robin over the 8 XCDs; remap so that each XCD
-  // walks one contiguous range of the tile order.
-  const int nb = gridDim.x;
-  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-  const int per = nb >> 3, rem = nb & 7;
-  const int tile = perm[(xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per) + slot];
-  const TileDesc* tp = tiles + tile;
-  const int I = tp->I, J = tp->J, e0 = tp->e0, ne = tp->ne, ktot = tp->ktot;
-  const int64_t out_base = tp->out_base;
-  const int natI = bands[I].natoms, natJ = bands[J].natoms;
-  const int32_t firstI = bands[I].first, firstJ = bands[J].first;
-
-  if (tid <= kBandAtoms) {
-    rs_r[tid] = bands[I].rs[min(tid, natI)];
-    rs_c[tid] = bands[J].rs[min(tid, natJ)];
-  }
-  if (tid < kBandAtoms) keptw[tid] = 0;
-  __syncthreads();
-  if (tid < kBandAtoms) {
-    uint32_t g = 0;
-    if (tid < natI && rs_r[tid + 1] > rs_r[tid])
-      for (int q = rs_r[tid] / 4; q <= (rs_r[tid + 1] - 1) / 4; ++q) g |= 1u << q;
-    pg_r[tid] = (uint16_t)g;
-    g = 0;
-    if (tid < natJ && rs_c[tid + 1] > rs_c[tid])
-      for (int q = rs_c[tid] / 4; q <= (rs_c[tid + 1] - 1) / 4; ++q) g |= 1u << q;
-    pg_c[tid] = (uint16_t)g;
-  }
-  // mask bit of the band atom holding this lane's operand rows (A) and
-  // columns (B); 0 for rows / columns past the band
-  uint32_t rbit[2] = {0u, 0u}, cbit[2] = {0u, 0u};
-  for (int a = 0; a < natI; ++a)
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int r = 32 * wr + 16 * h + (lane & 15);
-      if (r >= rs_r[a] && r < rs_r[a + 1]) rbit[h] = 1u << a;
-    }
-  for (int a = 0; a < natJ; ++a)
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int c = 32 * wc + 16 * q + (lane & 15);
-      if (c >= rs_c[a] && c < rs_c[a + 1]) cbit[q] = 1u << (16 + a);
-    }
+  const TileInfo T = tile_setup(tiles, perm, bands, sm);
+  const int e0 = T.e0, ne = T.ne, ktot = T.ktot;
+  uint32_t rbit[2], cbit[2];
+  lane_atom_bits(T, sm, wr, wc, lane, rbit, cbit);
   const int kc = lane >> 4;  // K column of this lane within a step
   // the 16-bit row / column group masks (4 rows or columns per group) of
   // this wave's halves and quads: rows 32wr+16h.., cols 32wc+16q..
@@ -226,7 +164,7 @@ __global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
   for (int Q0 = 0; Q0 < ktot; Q0 += kTableCols) {
     const int ncols = min(kTableCols, ktot - Q0);
     const int nsteps = (ncols + 3) >> 2;
-    __syncthreads();  // previous window done (and pg_r / pg_c visible)
+    __syncthreads();  // previous window done
     for (int t = tid; t < 4 * nsteps; t += 256) {
       col_ab[t] = make_uint2(a_zero, b_zero);
       col_m[t] = 0u;
@@ -240,13 +178,8 @@ __global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
       const int ke = (e + 1 < ne) ? entries[e0 + e + 1].kstart : ktot;
       const int lo = max(ks, Q0), hi = min(ke, Q0 + ncols);
       if (lo >= hi) continue;
-      uint32_t gr = 0, gc = 0;
-      for (int p = 0; p < kBandAtoms; ++p) {
-        if ((en.rmask >> p) & 1u) gr |= pg_r[p];
-        if ((en.cmask >> p) & 1u) gc |= pg_c[p];
-      }
       const uint32_t m = (uint32_t)en.rmask | ((uint32_t)en.cmask << 16);
-      const uint32_t g = gr | (gc << 16);
+      const uint32_t g = entry_groups(sm, en.rmask, en.cmask);
       for (int Q = lo; Q < hi; ++Q) {
         const int q = Q - Q0;
         col_ab[q] = make_uint2(en.a_col + (uint32_t)(Q - ks), en.b_col + (uint32_t)(Q - ks));
@@ -336,76 +269,9 @@ __global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
         const int col = 32 * wc + 16 * q + (lane & 15);
         Ct[row * kPitch + col] = acc[2 * h + q][r];
       }
-  // structural pairs (row-major over band positions) and their offsets
-  const int ip = tid >> 4, jp = tid & 15;
-  const bool present = ip < natI && jp < natJ && ((tp->pattern[ip] >> jp) & 1);
-  const int64_t sz =
-      present ? (int64_t)(rs_r[ip + 1] - rs_r[ip]) * (rs_c[jp + 1] - rs_c[jp]) : 0;
-  int cnt = present ? 1 : 0;
-  int64_t csz = sz;
-  for (int o = 1; o < 64; o <<= 1) {
-    const int c2 = __shfl_up(cnt, o, 64);
-    const int64_t s2 = __shfl_up(csz, o, 64);
-    if (lane >= o) {
-      cnt += c2;
-      csz += s2;
-    }
-  }
-  if (lane == 63) {
-    wave_cnt[wave] = cnt;
-    wave_sz[wave] = csz;
-  }
   __syncthreads();
-  int cbase = 0;
-  int64_t sbase = 0;
-  for (int w = 0; w < wave; ++w) {
-    cbase += wave_cnt[w];
-    sbase += wave_sz[w];
-  }
-  const int npairs = wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
-  if (present) {
-    pair_id[cbase + cnt - 1] = tid;
-    pair_off[cbase + cnt - 1] = sbase + csz - sz;
-  }
-  __syncthreads();
-
-  for (int pi = wave; pi < npairs; pi += 4) {
-    const int id = pair_id[pi];
-    const int pr = id >> 4, pc = id & 15;
-    const int m = rs_r[pr + 1] - rs_r[pr];
-    const int n = rs_c[pc + 1] - rs_c[pc];
-    const int len = m * n;
-    const int64_t base = out_base + pair_off[pi];
-    const double* old = nullptr;
-    if (beta != 0.0 && old_row_p != nullptr) {
-      const int32_t gi = firstI + pr, gj = firstJ + pc;
-      int64_t lo = old_row_p[gi], hi = old_row_p[gi + 1] - 1;
-      int64_t found = -1;
-      while (lo <= hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        const int32_t cc = old_col[mid];
-        if (cc == gj) {
-          found = mid;
-          break;
-        }
-        if (cc < gj) lo = mid + 1;
-        else hi = mid - 1;
-      }
-      if (found >= 0) old = old_data + old_off[found];
-    }
-    double part = 0.0;
-    for (int e = lane; e < len; e += 64) {
-      const int cc = e / m, r = e - cc * m;
-      double v = alpha * Ct[(rs_r[pr] + r) * kPitch + rs_c[pc] + cc];
-      if (old) v = v + beta * old[e];
-      c_data[base + e] = v;
-      part = __builtin_fma(v, v, part);
-    }
-    const double nsq = wave_sum(part);
-    if (lane == 0 && nsq >= eps2) atomicOr(&keptw[pr], 1u << pc);
-  }
-  __syncthreads();
-  if (tid < kBandAtoms) kept_out[(int64_t)tile * kBandAtoms + tid] = (uint16_t)keptw[tid];
+  tile_epilogue(Ct, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data, c_data,
+                kept_out);
 }
 
 hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* tiles,

@@ -131,9 +131,17 @@ extern "C" int32_t lamb_hip_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A_,
                      "multiply: A, B, C must share one distribution");
   if (C == A || C == B)
     return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_WRONG_ARGS, "multiply: C aliases A or B");
-  if (flags != LAMB_HIP_FP64)
+  const int mode = flags & 0xFF;
+  int slices = (flags >> 8) & 0xF;
+  if ((flags & ~0xFFF) != 0 || (mode != LAMB_HIP_FP64 && mode != LAMB_HIP_BF16_SPLIT) ||
+      (mode == LAMB_HIP_FP64 && slices != 0))
     return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_UNIMPLEMENTED,
-                     "multiply: flags=%d not implemented", flags);
+                     "multiply: flags=0x%x not implemented", flags);
+  const bool split = mode == LAMB_HIP_BF16_SPLIT;
+  if (split && slices == 0) slices = 5;
+  if (split && (slices < 3 || slices > 5))
+    return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_WRONG_ARGS,
+                     "multiply: BF16 split needs 3..5 slices, got %d", slices);
   if (d->max_blk > kBandRows)
     return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_UNIMPLEMENTED,
                      "multiply: block dimension %d above %d", d->max_blk, kBandRows);
@@ -156,20 +164,54 @@ extern "C" int32_t lamb_hip_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A_,
   HIPCHK(ctx, build_strips(s, S.W, MB, D, false, S.sb), "B strips");
   // panel columns are addressed by 32-bit indices; one zero column follows
   // the last panel of each side
-  if (S.sa.pk_total / 64 >= (int64_t)UINT32_MAX || S.sb.pk_total / 64 >= (int64_t)UINT32_MAX)
+  if (S.sa.pk_total / 64 >= (int64_t)UINT32_MAX || S.sb.pk_total / 64 >= (int64_t)UINT32_MAX ||
+      (int64_t)slices * (S.sa.pk_total / 512 + 1) * 64 >= (int64_t)INT32_MAX * 64)
     return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_OVERFLOW_ERROR,
                      "multiply: packed panels exceed 32-bit column indices");
-  const uint32_t a_zero = (uint32_t)(S.sa.pk_total / 64), b_zero = (uint32_t)(S.sb.pk_total / 64);
-  HIPCHK(ctx, ctx->d_apack.ensure(sizeof(double) * (size_t)(S.sa.pk_total + 64)), "a pack alloc");
-  HIPCHK(ctx, ctx->d_bpack.ensure(sizeof(double) * (size_t)(S.sb.pk_total + 64)), "b pack alloc");
-  HIPCHK(ctx, launch_pack_strips(s, S.sa.n, (const Strip*)S.sa.strips.p,
-                                 (const int64_t*)S.sa.pk.p, D.bands, D.blk,
-                                 (const double*)A->d_data.p, true, (double*)ctx->d_apack.p),
-         "pack A");
-  HIPCHK(ctx, launch_pack_strips(s, S.sb.n, (const Strip*)S.sb.strips.p,
-                                 (const int64_t*)S.sb.pk.p, D.bands, D.blk,
-                                 (const double*)B->d_data.p, false, (double*)ctx->d_bpack.p),
-         "pack B");
+  uint32_t a_zero, b_zero;
+  if (!split) {
+    a_zero = (uint32_t)(S.sa.pk_total / 64);
+    b_zero = (uint32_t)(S.sb.pk_total / 64);
+    HIPCHK(ctx, ctx->d_apack.ensure(sizeof(double) * (size_t)(S.sa.pk_total + 64)),
+           "a pack alloc");
+    HIPCHK(ctx, ctx->d_bpack.ensure(sizeof(double) * (size_t)(S.sb.pk_total + 64)),
+           "b pack alloc");
+    HIPCHK(ctx, launch_pack_strips(s, S.sa.n, (const Strip*)S.sa.strips.p,
+                                   (const int64_t*)S.sa.pk.p, D.bands, D.blk,
+                                   (const double*)A->d_data.p, true, (double*)ctx->d_apack.p),
+           "pack A");
+    HIPCHK(ctx, launch_pack_strips(s, S.sb.n, (const Strip*)S.sb.strips.p,
+                                   (const int64_t*)S.sb.pk.p, D.bands, D.blk,
+                                   (const double*)B->d_data.p, false, (double*)ctx->d_bpack.p),
+           "pack B");
+  } else {
+    // exponent per A row and B column, then S BF16 slice chunks per strip
+    // (chunks of 8 K columns; the chunk after the last one is zero)
+    const int64_t nca = S.sa.pk_total / 512, ncb = S.sb.pk_total / 512;
+    a_zero = (uint32_t)nca;
+    b_zero = (uint32_t)ncb;
+    const int64_t* elem0 = (const int64_t*)d->d_elem0.p;
+    HIPCHK(ctx, ctx->d_aexp.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(d->n_elem, 1)),
+           "a exponents alloc");
+    HIPCHK(ctx, ctx->d_bexp.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(d->n_elem, 1)),
+           "b exponents alloc");
+    HIPCHK(ctx, launch_slice_exponents(s, MA.nblk, MA.rows, MA.col, D.blk, elem0, MA.off,
+                                       (const double*)A->d_data.p, true, d->n_elem,
+                                       (int32_t*)ctx->d_aexp.p), "A exponents");
+    HIPCHK(ctx, launch_slice_exponents(s, MB.nblk, MB.rows, MB.col, D.blk, elem0, MB.off,
+                                       (const double*)B->d_data.p, false, d->n_elem,
+                                       (int32_t*)ctx->d_bexp.p), "B exponents");
+    HIPCHK(ctx, ctx->d_aslice.ensure((size_t)slices * (size_t)(nca + 1) * 64 * 16), "a slices");
+    HIPCHK(ctx, ctx->d_bslice.ensure((size_t)slices * (size_t)(ncb + 1) * 64 * 16), "b slices");
+    HIPCHK(ctx, launch_pack_slices(s, slices, S.sa.n, (const Strip*)S.sa.strips.p,
+                                   (const int64_t*)S.sa.pk.p, D.bands, D.blk, elem0,
+                                   (const int32_t*)ctx->d_aexp.p, (const double*)A->d_data.p,
+                                   true, nca, ctx->d_aslice.p), "slice A");
+    HIPCHK(ctx, launch_pack_slices(s, slices, S.sb.n, (const Strip*)S.sb.strips.p,
+                                   (const int64_t*)S.sb.pk.p, D.bands, D.blk, elem0,
+                                   (const int32_t*)ctx->d_bexp.p, (const double*)B->d_data.p,
+                                   false, ncb, ctx->d_bslice.p), "slice B");
+  }
   // ---- tiles and entries ----
   HIPCHK(ctx, build_tiles(s, S.W, D, S.sa, S.sb, use_old ? &MC : nullptr, eps2, S.r),
          "tiles");
@@ -192,15 +234,26 @@ extern "C" int32_t lamb_hip_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A_,
   HIPCHK(ctx, newdata.ensure(sizeof(double) * (size_t)std::max<int64_t>(S.r.c_size, 1)),
          "C pool");
   HIPCHK(ctx, hipEventRecord(ctx->ev[0], s), "event");
-  HIPCHK(ctx, launch_numeric_f64(s, n_tiles, (const TileDesc*)S.r.tiles.p,
-                                 (const int32_t*)S.r.perm.p,
-                                 (const Entry*)S.r.entries.p, D.bands,
-                                 (const double*)ctx->d_apack.p, (const double*)ctx->d_bpack.p,
-                                 a_zero, b_zero, alpha, beta, eps2, use_old ? MC.row_p : nullptr,
-                                 use_old ? MC.col : nullptr, use_old ? MC.off : nullptr,
-                                 use_old ? (const double*)C->d_data.p : nullptr,
-                                 (double*)newdata.p, (uint16_t*)ctx->d_kept.p),
-         "numeric kernel");
+  if (!split)
+    HIPCHK(ctx, launch_numeric_f64(s, n_tiles, (const TileDesc*)S.r.tiles.p,
+                                   (const int32_t*)S.r.perm.p,
+                                   (const Entry*)S.r.entries.p, D.bands,
+                                   (const double*)ctx->d_apack.p, (const double*)ctx->d_bpack.p,
+                                   a_zero, b_zero, alpha, beta, eps2,
+                                   use_old ? MC.row_p : nullptr, use_old ? MC.col : nullptr,
+                                   use_old ? MC.off : nullptr,
+                                   use_old ? (const double*)C->d_data.p : nullptr,
+                                   (double*)newdata.p, (uint16_t*)ctx->d_kept.p),
+           "numeric kernel");
+  else
+    HIPCHK(ctx, launch_numeric_bf16(s, slices, n_tiles, (const TileDesc*)S.r.tiles.p,
+                                    (const int32_t*)S.r.perm.p, (const Entry*)S.r.entries.p,
+                                    D.bands, ctx->d_aslice.p, ctx->d_bslice.p, a_zero, b_zero,
+                                    alpha, beta, eps2, use_old ? MC.row_p : nullptr,
+                                    use_old ? MC.col : nullptr, use_old ? MC.off : nullptr,
+                                    use_old ? (const double*)C->d_data.p : nullptr,
+                                    (double*)newdata.p, (uint16_t*)ctx->d_kept.p),
+           "numeric kernel (BF16 split)");
   HIPCHK(ctx, hipEventRecord(ctx->ev[1], s), "event");
 
   // ---- C index ----

@@ -118,11 +118,12 @@ __global__ void strip_fill(int64_t nblk, int a_side, const int64_t* __restrict__
             (unsigned long long)__double_as_longlong(nsq[b]));
 }
 
-// per-strip panel size (K columns x 64)
+// per-strip panel size: K columns rounded up to a multiple of 8 (one BF16
+// slice chunk), times 64
 __global__ void strip_panel_size(int64_t n, const Strip* __restrict__ s,
                                  const int32_t* __restrict__ blk, int64_t* __restrict__ sz) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) sz[i] = 64 * (int64_t)blk[s[i].k];
+  if (i < n) sz[i] = 64 * (int64_t)((blk[s[i].k] + 7) & ~7);
 }
 
 // ptr[g] = first strip of group g: strip index at the group's first slot

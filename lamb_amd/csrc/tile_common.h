@@ -1,0 +1,207 @@
+// Pieces shared by the numeric tile kernels (FP64 and BF16-split): tile
+// set-up, the per-lane atom masks of a wave's 32 x 32 quadrant, the live
+// block mask of a K step, and the epilogue (alpha / beta, block norms,
+// post-filter, C pool writes).
+#pragma once
+
+#include "common.h"
+
+namespace lamb {
+
+// LDS of one tile besides the main-loop table / C tile buffer
+struct TileSmem {
+  int16_t rs_r[kBandAtoms + 1], rs_c[kBandAtoms + 1];  // atom row / col starts
+  uint16_t pg_r[kBandAtoms], pg_c[kBandAtoms];          // 4-row / 4-col groups per atom
+  int32_t pair_id[kBandAtoms * kBandAtoms];
+  int64_t pair_off[kBandAtoms * kBandAtoms];
+  int32_t wave_cnt[4];
+  int64_t wave_sz[4];
+  uint32_t keptw[kBandAtoms];
+};
+
+struct TileInfo {
+  int tile, I, J, e0, ne, ktot, natI, natJ;
+  int32_t firstI, firstJ;
+  int64_t out_base;
+  const TileDesc* tp;
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Tile of this workgroup and its band geometry in LDS. Workgroups are dealt
+// round-robin over the 8 XCDs; the remap lets each XCD walk one contiguous
+// range of the tile order perm. Ends with the LDS visible to all threads.
+__device__ __forceinline__ TileInfo tile_setup(const TileDesc* __restrict__ tiles,
+                                               const int32_t* __restrict__ perm,
+                                               const Band* __restrict__ bands, TileSmem& sm) {
+  const int tid = threadIdx.x;
+  const int nb = gridDim.x;
+  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int per = nb >> 3, rem = nb & 7;
+  TileInfo t;
+  t.tile = perm[(xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per) + slot];
+  t.tp = tiles + t.tile;
+  t.I = t.tp->I;
+  t.J = t.tp->J;
+  t.e0 = t.tp->e0;
+  t.ne = t.tp->ne;
+  t.ktot = t.tp->ktot;
+  t.out_base = t.tp->out_base;
+  t.natI = bands[t.I].natoms;
+  t.natJ = bands[t.J].natoms;
+  t.firstI = bands[t.I].first;
+  t.firstJ = bands[t.J].first;
+  if (tid <= kBandAtoms) {
+    sm.rs_r[tid] = bands[t.I].rs[min(tid, t.natI)];
+    sm.rs_c[tid] = bands[t.J].rs[min(tid, t.natJ)];
+  }
+  if (tid < kBandAtoms) sm.keptw[tid] = 0;
+  __syncthreads();
+  if (tid < kBandAtoms) {
+    uint32_t g = 0;
+    if (tid < t.natI && sm.rs_r[tid + 1] > sm.rs_r[tid])
+      for (int q = sm.rs_r[tid] / 4; q <= (sm.rs_r[tid + 1] - 1) / 4; ++q) g |= 1u << q;
+    sm.pg_r[tid] = (uint16_t)g;
+    g = 0;
+    if (tid < t.natJ && sm.rs_c[tid + 1] > sm.rs_c[tid])
+      for (int q = sm.rs_c[tid] / 4; q <= (sm.rs_c[tid + 1] - 1) / 4; ++q) g |= 1u << q;
+    sm.pg_c[tid] = (uint16_t)g;
+  }
+  __syncthreads();
+  return t;
+}
+
+// Mask bit of the band atom holding this lane's operand rows (A, bits 0..15)
+// and columns (B, bits 16..31) for the quadrant of wave (wr, wc): rows
+// 32wr + 16h + (lane & 15), cols 32wc + 16q + (lane & 15); 0 past the band.
+__device__ __forceinline__ void lane_atom_bits(const TileInfo& t, const TileSmem& sm, int wr,
+                                               int wc, int lane, uint32_t rbit[2],
+                                               uint32_t cbit[2]) {
+  rbit[0] = rbit[1] = cbit[0] = cbit[1] = 0u;
+  for (int a = 0; a < t.natI; ++a)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int r = 32 * wr + 16 * h + (lane & 15);
+      if (r >= sm.rs_r[a] && r < sm.rs_r[a + 1]) rbit[h] = 1u << a;
+    }
+  for (int a = 0; a < t.natJ; ++a)
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int c = 32 * wc + 16 * q + (lane & 15);
+      if (c >= sm.rs_c[a] && c < sm.rs_c[a + 1]) cbit[q] = 1u << (16 + a);
+    }
+}
+
+// Row / column 4-groups touched by an entry with atom masks rmask, cmask:
+// rows in the low 16 bits, columns in the high 16 bits.
+__device__ __forceinline__ uint32_t entry_groups(const TileSmem& sm, uint32_t rmask,
+                                                 uint32_t cmask) {
+  uint32_t gr = 0, gc = 0;
+  for (int p = 0; p < kBandAtoms; ++p) {
+    if ((rmask >> p) & 1u) gr |= sm.pg_r[p];
+    if ((cmask >> p) & 1u) gc |= sm.pg_c[p];
+  }
+  return gr | (gc << 16);
+}
+
+// 4-bit mask of the live 16 x 16 blocks of a wave's 32 x 32 quadrant in a
+// step whose row / column 4-groups are g (rows low half, columns high half):
+// bit 2h+q = row half h and column half q both hold a live group.
+__device__ __forceinline__ uint32_t quadrant_blocks(uint32_t g, uint32_t gsh_r, uint32_t gsh_c) {
+  const uint32_t rg = (g & 0xFFFFu) >> gsh_r, cg = (g >> 16) >> gsh_c;
+  const uint32_t r0 = (rg & 0xFu) ? 1u : 0u, r1 = (rg & 0xF0u) ? 1u : 0u;
+  const uint32_t c0 = (cg & 0xFu) ? 1u : 0u, c1 = (cg & 0xF0u) ? 1u : 0u;
+  return (r0 & c0) | ((r0 & c1) << 1) | ((r1 & c0) << 2) | ((r1 & c1) << 3);
+}
+
+// Epilogue. Ct holds the tile's 64 x 64 accumulated products (row pitch
+// kPitch) and is visible to all threads. Each structural block (i,j) of the
+// tile becomes alpha * (A*B)_ij + beta * Cold_ij (Cold found by binary search
+// in its CSR row), is written to the C pool at out_base + its offset
+// (blocks back to back in row-major order of atom pairs, each column-major),
+// and is kept iff its sum of squares (sequential FMA order within a lane,
+// then a wave sum) is >= eps2. kept_out[tile*16 + i] gets the kept columns
+// of band atom i.
+__device__ __forceinline__ void tile_epilogue(
+    const double* Ct, const TileInfo& t, TileSmem& sm, double alpha, double beta, double eps2,
+    const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
+    const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
+    double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  // structural pairs (row-major over band positions) and their offsets
+  const int ip = tid >> 4, jp = tid & 15;
+  const bool present = ip < t.natI && jp < t.natJ && ((t.tp->pattern[ip] >> jp) & 1);
+  const int64_t sz =
+      present ? (int64_t)(sm.rs_r[ip + 1] - sm.rs_r[ip]) * (sm.rs_c[jp + 1] - sm.rs_c[jp]) : 0;
+  int cnt = present ? 1 : 0;
+  int64_t csz = sz;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int c2 = __shfl_up(cnt, o, 64);
+    const int64_t s2 = __shfl_up(csz, o, 64);
+    if (lane >= o) {
+      cnt += c2;
+      csz += s2;
+    }
+  }
+  if (lane == 63) {
+    sm.wave_cnt[wave] = cnt;
+    sm.wave_sz[wave] = csz;
+  }
+  __syncthreads();
+  int cbase = 0;
+  int64_t sbase = 0;
+  for (int w = 0; w < wave; ++w) {
+    cbase += sm.wave_cnt[w];
+    sbase += sm.wave_sz[w];
+  }
+  const int npairs = sm.wave_cnt[0] + sm.wave_cnt[1] + sm.wave_cnt[2] + sm.wave_cnt[3];
+  if (present) {
+    sm.pair_id[cbase + cnt - 1] = tid;
+    sm.pair_off[cbase + cnt - 1] = sbase + csz - sz;
+  }
+  __syncthreads();
+
+  for (int pi = wave; pi < npairs; pi += 4) {
+    const int id = sm.pair_id[pi];
+    const int pr = id >> 4, pc = id & 15;
+    const int m = sm.rs_r[pr + 1] - sm.rs_r[pr];
+    const int n = sm.rs_c[pc + 1] - sm.rs_c[pc];
+    const int len = m * n;
+    const int64_t base = t.out_base + sm.pair_off[pi];
+    const double* old = nullptr;
+    if (beta != 0.0 && old_row_p != nullptr) {
+      const int32_t gi = t.firstI + pr, gj = t.firstJ + pc;
+      int64_t lo = old_row_p[gi], hi = old_row_p[gi + 1] - 1;
+      int64_t found = -1;
+      while (lo <= hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        const int32_t cc = old_col[mid];
+        if (cc == gj) {
+          found = mid;
+          break;
+        }
+        if (cc < gj) lo = mid + 1;
+        else hi = mid - 1;
+      }
+      if (found >= 0) old = old_data + old_off[found];
+    }
+    double part = 0.0;
+    for (int e = lane; e < len; e += 64) {
+      const int cc = e / m, r = e - cc * m;
+      double v = alpha * Ct[(sm.rs_r[pr] + r) * kPitch + sm.rs_c[pc] + cc];
+      if (old) v = v + beta * old[e];
+      c_data[base + e] = v;
+      part = __builtin_fma(v, v, part);
+    }
+    const double nsq = wave_sum(part);
+    if (lane == 0 && nsq >= eps2) atomicOr(&sm.keptw[pr], 1u << pc);
+  }
+  __syncthreads();
+  if (tid < kBandAtoms) kept_out[(int64_t)t.tile * kBandAtoms + tid] = (uint16_t)sm.keptw[tid];
+}
+
+}  // namespace lamb

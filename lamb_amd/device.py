@@ -14,6 +14,11 @@ FP64 = 0
 BF16_SPLIT = 1
 
 
+def bf16_split(slices: int = 5) -> int:
+    """multiply flags for the BF16-split form with the given slice count (3..5)."""
+    return BF16_SPLIT | (int(slices) << 8)
+
+
 class LambError(RuntimeError):
     def __init__(self, level: int, type_: int, msg: str):
         super().__init__(f"[{LEVELS.get(level, level)} / type {type_}] {msg}")
