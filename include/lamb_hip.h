@@ -156,6 +156,36 @@ int32_t lamb_hip_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A, const lamb_h
 /* tr(A): sum of the diagonal elements of the diagonal blocks (FP64). */
 int32_t lamb_hip_trace(const lamb_hip_mat* A, double* trace);
 
+/* --- batches and purification chains --------------------------------------
+ * None of these has a counterpart in the reference, whose multiply computes
+ * nothing; they are the matrix functions a McWeeny purification chain needs
+ * (P <- 3P^2 - 2P^3, started from H scaled into [0,1]). A batch of
+ * independent systems is one distribution whose atoms carry a group (system)
+ * id: matrices are block diagonal over groups, every multiply runs all
+ * groups in one launch per kernel, and these functions report per group.
+ * Results are FP64 and deterministic (per-atom partials summed in order). */
+/* group_of_atom[g] in 0..n_groups-1 for each global atom g (default: all 0). */
+int32_t lamb_hip_dist_set_groups(lamb_hip_dist* dist, int32_t n_groups,
+                                 const int32_t* group_of_atom);
+/* out[n_groups]: per-group trace, Frobenius norm, ||X - Y||_F (union pattern). */
+int32_t lamb_hip_group_trace(const lamb_hip_mat* A, double* out);
+int32_t lamb_hip_group_norm(const lamb_hip_mat* A, double* out);
+int32_t lamb_hip_group_diff_norm(const lamb_hip_mat* X, const lamb_hip_mat* Y, double* out);
+/* Per-group Gershgorin bounds of the spectrum: lo = min_r (H_rr - R_r),
+ * hi = max_r (H_rr + R_r), R_r = sum over c != r of |H_rc|. Warning level
+ * (precondition) when an atom has no stored diagonal block. */
+int32_t lamb_hip_group_gershgorin(const lamb_hip_mat* A, double* lo, double* hi);
+/* A <- a[g] * A + b[g] * I per group; every atom needs its diagonal block
+ * when some b[g] != 0 (else failure, precondition). */
+int32_t lamb_hip_group_scale_shift(lamb_hip_mat* A, const double* a, const double* b);
+/* steps McWeeny iterations on P in place, every multiply filtered with
+ * eps_filter and computed in the form flags selects. Step k writes
+ * idem[k*n_groups + g] = ||P^2 - P||_F of its input and
+ * traces[k*n_groups + g] = tr of its output (either may be NULL). stats
+ * accumulates the 2*steps multiplies. */
+int32_t lamb_hip_mcweeny(lamb_hip_ctx* ctx, lamb_hip_mat* P, int32_t steps, double eps_filter,
+                         int32_t flags, double* traces, double* idem, lamb_hip_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,14 @@ SIGNATURES = {
                                       C.c_double, C.c_double, C.c_double, C.c_int32,
                                       C.POINTER(Stats)]),
     "lamb_hip_trace": (C.c_int32, [C.c_void_p, f64p]),
+    "lamb_hip_dist_set_groups": (C.c_int32, [C.c_void_p, C.c_int32, i32p]),
+    "lamb_hip_group_trace": (C.c_int32, [C.c_void_p, f64p]),
+    "lamb_hip_group_norm": (C.c_int32, [C.c_void_p, f64p]),
+    "lamb_hip_group_diff_norm": (C.c_int32, [C.c_void_p, C.c_void_p, f64p]),
+    "lamb_hip_group_gershgorin": (C.c_int32, [C.c_void_p, f64p, f64p]),
+    "lamb_hip_group_scale_shift": (C.c_int32, [C.c_void_p, f64p, f64p]),
+    "lamb_hip_mcweeny": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_int32,
+                                     f64p, f64p, C.POINTER(Stats)]),
 }
 
 _lib = None

@@ -219,6 +219,10 @@ int32_t lamb_hip_dist_create(lamb_hip_ctx* ctx, int64_t n_atoms, const int8_t* b
   d->n_elem = elem0[d->n];
   HIPCHK(ctx, d->d_elem0.upload(ctx->stream, elem0.data(), sizeof(int64_t) * (d->n + 1)),
          "dist elem0");
+  d->n_groups = 1;
+  d->group_int.assign(d->n, 0);
+  HIPCHK(ctx, d->d_group.upload(ctx->stream, d->group_int.data(), sizeof(int32_t) * d->n),
+         "dist groups");
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "dist sync");
   *out = d.release();
   return LAMB_HIP_OK;
@@ -232,6 +236,7 @@ int32_t lamb_hip_dist_destroy(lamb_hip_dist* d) {
   d->d_band_of.release();
   d->d_pos_in_band.release();
   d->d_elem0.release();
+  d->d_group.release();
   delete d;
   return LAMB_HIP_OK;
 }

@@ -80,6 +80,10 @@ struct lamb_hip_dist {
   lamb::DevBuf d_band_of, d_pos_in_band;  // int32 by internal atom
   lamb::DevBuf d_elem0;  // int64 [n+1]: first matrix row of each internal atom
   int64_t n_elem = 0;    // matrix dimension (sum of block sizes)
+  // groups of atoms (independent systems of a batch), by internal atom
+  int32_t n_groups = 1;
+  std::vector<int32_t> group_int;
+  lamb::DevBuf d_group;
 };
 
 struct lamb_hip_mat {

@@ -75,6 +75,15 @@ class DeviceDistribution:
                                           N.ptr(box, C.c_float), C.byref(h))
         ctx.check(rc, "dist_create")
         self.h = h
+        self.n_groups = 1
+
+    def set_groups(self, group_of_atom: np.ndarray, n_groups: int | None = None):
+        """Group (independent system) id in 0..n_groups-1 of each global atom."""
+        g = np.ascontiguousarray(group_of_atom, dtype=np.int32)
+        n = int(g.max()) + 1 if n_groups is None else int(n_groups)
+        self.ctx.check(self.ctx.lib.lamb_hip_dist_set_groups(self.h, n, N.ptr(g, C.c_int32)),
+                       "dist_set_groups")
+        self.n_groups = n
 
     def close(self):
         if self.h:
@@ -145,10 +154,64 @@ class DeviceMatrix:
         self.ctx.check(self.ctx.lib.lamb_hip_trace(self.h, C.byref(t)), "trace")
         return t.value
 
+    def _per_group(self, fn, *args) -> np.ndarray:
+        out = np.zeros(self.ddist.n_groups)
+        self.ctx.check(fn(self.h, *args, N.ptr(out, C.c_double)), fn.__name__)
+        return out
+
+    def group_trace(self) -> np.ndarray:
+        return self._per_group(self.ctx.lib.lamb_hip_group_trace)
+
+    def group_norm(self) -> np.ndarray:
+        return self._per_group(self.ctx.lib.lamb_hip_group_norm)
+
+    def group_diff_norm(self, other: "DeviceMatrix") -> np.ndarray:
+        return self._per_group(self.ctx.lib.lamb_hip_group_diff_norm, other.h)
+
+    def group_gershgorin(self):
+        lo = np.zeros(self.ddist.n_groups)
+        hi = np.zeros(self.ddist.n_groups)
+        rc = self.ctx.lib.lamb_hip_group_gershgorin(self.h, N.ptr(lo, C.c_double),
+                                                    N.ptr(hi, C.c_double))
+        self.ctx.check(rc, "group_gershgorin")
+        return lo, hi
+
+    def group_scale_shift(self, a, b):
+        a = np.ascontiguousarray(np.broadcast_to(a, self.ddist.n_groups), dtype=np.float64)
+        b = np.ascontiguousarray(np.broadcast_to(b, self.ddist.n_groups), dtype=np.float64)
+        rc = self.ctx.lib.lamb_hip_group_scale_shift(self.h, N.ptr(a, C.c_double),
+                                                     N.ptr(b, C.c_double))
+        self.ctx.check(rc, "group_scale_shift")
+
     def close(self):
         if self.h:
             self.ctx.lib.lamb_hip_matrix_destroy(self.h)
             self.h = None
+
+
+def purification_start(h: DeviceMatrix) -> DeviceMatrix:
+    """P0 = (H - lo I) / (hi - lo) per group, lo / hi the Gershgorin bounds."""
+    lo, hi = h.group_gershgorin()
+    p = DeviceMatrix(h.ddist)
+    p.copy_from(h)
+    span = hi - lo
+    p.group_scale_shift(1.0 / span, -lo / span)
+    return p
+
+
+def mcweeny(p: DeviceMatrix, steps: int, *, eps_filter: float = 0.0, flags: int = FP64):
+    """steps McWeeny iterations on p in place; returns (traces, idempotency
+    errors), both [steps, n_groups], and the accumulated statistics."""
+    ctx = p.ctx
+    g = p.ddist.n_groups
+    traces = np.zeros((steps, g))
+    idem = np.zeros((steps, g))
+    st = N.Stats()
+    rc = ctx.lib.lamb_hip_mcweeny(ctx.h, p.h, int(steps), float(eps_filter), int(flags),
+                                  N.ptr(traces, C.c_double), N.ptr(idem, C.c_double),
+                                  C.byref(st))
+    ctx.check(rc, "mcweeny")
+    return traces, idem, st.as_dict()
 
 
 def multiply(a: DeviceMatrix, b: DeviceMatrix, c: DeviceMatrix, *, alpha: float = 1.0,

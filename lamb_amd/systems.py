@@ -242,3 +242,89 @@ def small_system(n_atoms: int, seed: int, box: float | None = None,
     m = decay_matrix(atoms, cutoff, decay, seed + 7, symmetric=symmetric,
                      diag_shift=2.0 if symmetric else 0.0)
     return atoms, m
+
+
+# --- batches of independent systems ----------------------------------------
+
+@dataclasses.dataclass
+class Batch:
+    """Independent systems as one block-diagonal problem.
+
+    The systems' boxes are laid side by side along x with a gap, so the
+    union's coordinates keep each system compact (spatial orderings never
+    mix two systems); ``group`` is the system of each atom and ``h`` the
+    block-diagonal matrix of the systems' matrices.
+    """
+
+    atoms: AtomBox
+    group: np.ndarray  # int32 [n]
+    systems: list      # [(AtomBox, BlockCSR)] per system
+    h: BlockCSR
+
+    @property
+    def n_systems(self) -> int:
+        return len(self.systems)
+
+    def split(self, m: BlockCSR) -> list:
+        """Per-system diagonal blocks of a block-diagonal batch matrix."""
+        out = []
+        start = 0
+        for atoms, _ in self.systems:
+            n = atoms.n_atoms
+            r0, r1 = int(m.row_p[start]), int(m.row_p[start + n])
+            cols = m.col[r0:r1] - start
+            assert cols.size == 0 or (cols.min() >= 0 and cols.max() < n)
+            offs = m.blk_off[r0:r1]
+            sizes = m.blk[m.rows()[r0:r1]].astype(np.int64) * m.blk[m.col[r0:r1]]
+            data = np.concatenate([m.data[o:o + z] for o, z in zip(offs, sizes)]) \
+                if r1 > r0 else np.zeros(0)
+            out.append(BlockCSR(atoms.blk.copy(), (m.row_p[start:start + n + 1] - r0).astype(np.int64),
+                                cols.astype(np.int32), _offsets(sizes)[:-1], data))
+            start += n
+        return out
+
+
+def batch(systems: list, gap: float = 20.0) -> Batch:
+    """Block-diagonal batch of (AtomBox, BlockCSR) systems."""
+    pos, blk, group = [], [], []
+    rows, cols, offs, datas = [np.zeros(1, np.int64)], [], [], []
+    shift = 0.0
+    a0 = 0
+    d0 = 0
+    for g, (atoms, m) in enumerate(systems):
+        p = atoms.pos.astype(np.float64).copy()
+        p[:, 0] += shift
+        shift += atoms.box + gap
+        pos.append(p)
+        blk.append(atoms.blk)
+        group.append(np.full(atoms.n_atoms, g, np.int32))
+        rows.append(m.row_p[1:] + (rows[-1][-1] if len(rows) > 1 else 0))
+        cols.append(m.col + a0)
+        offs.append(m.blk_off + d0)
+        datas.append(m.data)
+        a0 += atoms.n_atoms
+        d0 += m.data.size
+    L = float(np.ceil(max(shift, max(a.box for a, _ in systems))))
+    allpos = np.concatenate(pos)
+    atoms = AtomBox(_wrap32(allpos, L), np.concatenate(blk).astype(np.int32), L)
+    row_p = np.concatenate(rows).astype(np.int64)
+    h = BlockCSR(atoms.blk.copy(), row_p, np.concatenate(cols).astype(np.int32),
+                 np.concatenate(offs).astype(np.int64), np.concatenate(datas))
+    return Batch(atoms, np.concatenate(group), list(systems), h)
+
+
+def config4(seed: int = 4, n_systems: int = 64, lo: int = 96, hi: int = 3072,
+            heavy_fraction: float = 0.1, cutoff: float = 8.0, decay: float = 0.4):
+    """Batch of water-like systems with log-uniform sizes in [lo, hi] atoms
+    (density 0.1 / A^3), O 13 / H 5 blocks and 23 x 23 blocks for
+    ``heavy_fraction`` of the atoms, each with a symmetric H
+    (U[-1,1] exp(-decay r), r < cutoff)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), n_systems)).round().astype(int)
+    systems = []
+    for s, n in enumerate(sizes):
+        atoms = water_system(int(n), seed * 1000 + s, cutoff, heavy_fraction)
+        h = decay_matrix(atoms, cutoff, decay, seed * 1000 + 500 + s, symmetric=True)
+        systems.append((atoms, h))
+    return batch(systems)
+

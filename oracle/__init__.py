@@ -164,3 +164,56 @@ def adjoint(a, b, g):
     lib().orc_adjoint(C.byref(ma), C.byref(mb), C.byref(mg), _p(da, C.c_double),
                       _p(db, C.c_double))
     return da[:len(a.data)], db[:len(b.data)]
+
+
+def scale_shift(csr, a: float, b: float):
+    """a * X + b * I (every diagonal block must be stored when b != 0)."""
+    from lamb_amd.systems import BlockCSR
+    data = a * csr.data
+    if b != 0.0:
+        rows = csr.rows()
+        for k in np.flatnonzero(rows == csr.col):
+            m = int(csr.blk[rows[k]])
+            idx = csr.blk_off[k] + np.arange(m) * (m + 1)
+            data[idx] += b
+        missing = set(np.flatnonzero(csr.blk > 0)) - set(rows[rows == csr.col])
+        if missing:
+            raise ValueError(f"{len(missing)} atoms without a stored diagonal block")
+    return BlockCSR(csr.blk.copy(), csr.row_p.copy(), csr.col.copy(), csr.blk_off.copy(), data)
+
+
+def purification_start(h):
+    """P0 = (H - lo I) / (hi - lo), lo / hi the Gershgorin bounds of H."""
+    lo, hi = gershgorin(h)
+    return scale_shift(h, 1.0 / (hi - lo), -lo / (hi - lo))
+
+
+def diff_norm(x, y) -> float:
+    """||X - Y||_F over the union pattern (blocks compared by (row, col))."""
+    def blocks(m):
+        rows = m.rows()
+        out = {}
+        for k in range(m.n_blocks):
+            z = int(m.blk[rows[k]]) * int(m.blk[m.col[k]])
+            out[(int(rows[k]), int(m.col[k]))] = m.data[m.blk_off[k]:m.blk_off[k] + z]
+        return out
+    bx, by = blocks(x), blocks(y)
+    s = 0.0
+    for key in set(bx) | set(by):
+        d = bx.get(key, 0.0) - by.get(key, 0.0)
+        s += float(np.sum(np.square(d)))
+    return float(np.sqrt(s))
+
+
+def mcweeny(p, steps: int, *, eps: float = 0.0):
+    """P <- 3 P^2 - 2 P^3, steps times, with the same two filtered multiplies
+    per step as the device chain (P2 = P*P; T = -2 P2*P + 3 P2). Returns
+    (P, traces [steps], idempotency errors ||P^2 - P||_F of each step's
+    input [steps])."""
+    traces, idem = [], []
+    for _ in range(steps):
+        p2, _ = multiply(p, p, eps=eps)
+        idem.append(diff_norm(p2, p))
+        p, _ = multiply(p2, p, eps=eps, alpha=-2.0, beta=3.0, c_old=p2)
+        traces.append(trace(p))
+    return p, np.array(traces), np.array(idem)
