@@ -186,6 +186,32 @@ int32_t lamb_hip_group_scale_shift(lamb_hip_mat* A, const double* a, const doubl
 int32_t lamb_hip_mcweeny(lamb_hip_ctx* ctx, lamb_hip_mat* P, int32_t steps, double eps_filter,
                          int32_t flags, double* traces, double* idem, lamb_hip_stats* stats);
 
+/* --- differentiation ------------------------------------------------------
+ * No counterpart in the reference. Reverse-mode adjoint of C = A*B: with G
+ * the gradient of a loss with respect to C (stored on C's pattern),
+ *   dA = (G B^T) sampled on A's pattern,  dB = (A^T G) sampled on B's pattern,
+ * each an SDDMM computed as one masked multiply by an explicit transpose, in
+ * the form flags selects (FP64 or BF16 split), unfiltered. dA / dB (either
+ * may be NULL) get exactly A's / B's pattern. */
+int32_t lamb_hip_multiply_adjoint(lamb_hip_ctx* ctx, const lamb_hip_mat* A,
+                                  const lamb_hip_mat* B, const lamb_hip_mat* G,
+                                  lamb_hip_mat* dA, lamb_hip_mat* dB, int32_t flags,
+                                  lamb_hip_stats* stats);
+/* XT = X^T (distinct matrices of one distribution). */
+int32_t lamb_hip_matrix_transpose(const lamb_hip_mat* X, lamb_hip_mat* XT);
+/* dst = like's pattern holding src's blocks (zero blocks where src has none). */
+int32_t lamb_hip_matrix_project(const lamb_hip_mat* src, const lamb_hip_mat* like,
+                                lamb_hip_mat* dst);
+/* Values in canonical order (stored blocks in CSR order of the internal atom
+ * order, each column-major, back to back; n = n_elems of matrix_info) to or
+ * from DEVICE memory of the library's device. */
+int32_t lamb_hip_matrix_get_values(const lamb_hip_mat* X, double* dev_values, int64_t n);
+int32_t lamb_hip_matrix_set_values(lamb_hip_mat* X, const double* dev_values, int64_t n);
+/* Frees X's value pool, keeping its pattern (set_values brings it back). */
+int32_t lamb_hip_matrix_release_values(lamb_hip_mat* X);
+/* Global (0-based) row and column atom of each stored block, canonical order. */
+int32_t lamb_hip_matrix_block_atoms(lamb_hip_mat* X, int32_t* rows, int32_t* cols);
+
 #ifdef __cplusplus
 }
 #endif

@@ -72,6 +72,15 @@ SIGNATURES = {
     "lamb_hip_group_scale_shift": (C.c_int32, [C.c_void_p, f64p, f64p]),
     "lamb_hip_mcweeny": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_int32,
                                      f64p, f64p, C.POINTER(Stats)]),
+    "lamb_hip_multiply_adjoint": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_void_p, C.c_void_p, C.c_int32,
+                                              C.POINTER(Stats)]),
+    "lamb_hip_matrix_transpose": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "lamb_hip_matrix_project": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "lamb_hip_matrix_get_values": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64]),
+    "lamb_hip_matrix_set_values": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64]),
+    "lamb_hip_matrix_release_values": (C.c_int32, [C.c_void_p]),
+    "lamb_hip_matrix_block_atoms": (C.c_int32, [C.c_void_p, i32p, i32p]),
 }
 
 _lib = None

@@ -284,6 +284,7 @@ int32_t lamb_hip_matrix_copy(const lamb_hip_mat* src, lamb_hip_mat* dst) {
     return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_WRONG_ARGS,
                      "matrix_copy: matrices have different distributions");
   if (src == dst) return LAMB_HIP_OK;
+  if (values_missing(src)) return refuse_missing_values(ctx, "matrix_copy");
   (void)hipSetDevice(ctx->device);
   hipStream_t s = ctx->stream;
   auto d2d = [&](DevBuf& to, const DevBuf& from, size_t bytes) -> hipError_t {
@@ -575,6 +576,7 @@ int32_t lamb_hip_sector_download(lamb_hip_mat* m, int32_t sr, int32_t sc, int64_
   if (sr < 1 || sr > d->n_subsets || sc < 1 || sc > d->n_subsets)
     return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_WRONG_ARGS,
                      "sector_download: subsets (%d,%d) outside 1..%d", sr, sc, d->n_subsets);
+  if (values_missing(m)) return refuse_missing_values(ctx, "sector_download");
   (void)hipSetDevice(ctx->device);
   HIPCHK(ctx, m->download_index(ctx->stream), "index download");
   int32_t rc = ensure_host_data(m);
@@ -603,6 +605,7 @@ int32_t lamb_hip_trace(const lamb_hip_mat* m, double* trace) {
   if (!m || !trace) return LAMB_HIP_FAILURE;
   lamb_hip_dist* d = m->dist;
   lamb_hip_ctx* ctx = d->ctx;
+  if (values_missing(m)) return refuse_missing_values(ctx, "trace");
   (void)hipSetDevice(ctx->device);
   DevBuf part;
   HIPCHK(ctx, part.ensure(sizeof(double) * std::max(d->n, 1)), "trace alloc");

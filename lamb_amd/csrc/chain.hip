@@ -167,6 +167,7 @@ int32_t atom_partials(lamb_hip_ctx* ctx, std::vector<double>& host, DevBuf& dev,
 int32_t group_trace(lamb_hip_mat* A, double* out) {
   lamb_hip_dist* d = A->dist;
   lamb_hip_ctx* ctx = d->ctx;
+  if (values_missing(A)) return refuse_missing_values(ctx, "trace");
   DevBuf part;
   HIPCHK(ctx, part.ensure(sizeof(double) * std::max(d->n, 1)), "trace partials");
   HIPCHK(ctx, launch_trace(ctx->stream, d->n, (const int64_t*)A->d_row_p.p,
@@ -184,6 +185,7 @@ int32_t group_trace(lamb_hip_mat* A, double* out) {
 int32_t group_diff_sumsq(lamb_hip_mat* X, lamb_hip_mat* Y, double* out) {
   lamb_hip_dist* d = X->dist;
   lamb_hip_ctx* ctx = d->ctx;
+  if (values_missing(X) || (Y && values_missing(Y))) return refuse_missing_values(ctx, "norm");
   DevBuf part;
   HIPCHK(ctx, part.ensure(sizeof(double) * std::max(d->n, 1)), "norm partials");
   if (d->n) {
@@ -266,6 +268,7 @@ extern "C" int32_t lamb_hip_group_gershgorin(const lamb_hip_mat* A, double* lo, 
   if (!A || !lo || !hi) return LAMB_HIP_FAILURE;
   lamb_hip_dist* d = A->dist;
   lamb_hip_ctx* ctx = d->ctx;
+  if (values_missing(A)) return refuse_missing_values(ctx, "gershgorin");
   (void)hipSetDevice(ctx->device);
   DevBuf plo, phi, miss;
   HIPCHK(ctx, plo.ensure(sizeof(double) * std::max(d->n, 1)), "gershgorin partials");
@@ -310,6 +313,7 @@ extern "C" int32_t lamb_hip_group_scale_shift(lamb_hip_mat* A, const double* a, 
   if (!A || !a || !b) return LAMB_HIP_FAILURE;
   lamb_hip_dist* d = A->dist;
   lamb_hip_ctx* ctx = d->ctx;
+  if (values_missing(A)) return refuse_missing_values(ctx, "scale_shift");
   (void)hipSetDevice(ctx->device);
   // the shift needs every diagonal block of an atom with rows
   bool has_shift = false;

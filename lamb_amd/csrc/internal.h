@@ -38,6 +38,17 @@ struct StagedList {
 int32_t set_error(lamb_hip_ctx* ctx, int32_t level, int32_t type, const char* fmt, ...);
 int32_t hip_error(lamb_hip_ctx* ctx, hipError_t e, const char* what);
 
+// lamb_hip_multiply restricted to the blocks of mask's pattern (mask may be
+// nullptr: the plain multiply).
+int32_t multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A, const lamb_hip_mat* B,
+                        lamb_hip_mat* C, double alpha, double beta, double eps, int32_t flags,
+                        lamb_hip_stats* stats, const lamb_hip_mat* mask);
+
+// A matrix whose value pool was released (lamb_hip_matrix_release_values)
+// while it stores blocks: entry points that read values refuse it.
+bool values_missing(const lamb_hip_mat* m);
+int32_t refuse_missing_values(lamb_hip_ctx* ctx, const char* what);
+
 // Releases the device scratch of the symbolic phase held by a context.
 void release_sym_state(lamb_hip_ctx* ctx);
 

@@ -95,6 +95,15 @@ static int32_t ensure_nsq(lamb_hip_mat* m) {
   return LAMB_HIP_OK;
 }
 
+bool lamb::values_missing(const lamb_hip_mat* m) {
+  return m->nblk > 0 && m->ndata > 0 && m->d_data.p == nullptr;
+}
+
+int32_t lamb::refuse_missing_values(lamb_hip_ctx* ctx, const char* what) {
+  return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_PRECONDITION_FAILED,
+                   "%s: the matrix's values were released", what);
+}
+
 static DistView dist_view(const lamb_hip_dist* d) {
   DistView v;
   v.n = d->n;
@@ -117,10 +126,16 @@ static MatView mat_view(const lamb_hip_mat* m) {
   return v;
 }
 
-extern "C" int32_t lamb_hip_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A_,
-                                     const lamb_hip_mat* B_, lamb_hip_mat* C, double alpha,
+extern "C" int32_t lamb_hip_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A,
+                                     const lamb_hip_mat* B, lamb_hip_mat* C, double alpha,
                                      double beta, double eps, int32_t flags,
                                      lamb_hip_stats* stats) {
+  return lamb::multiply_masked(ctx, A, B, C, alpha, beta, eps, flags, stats, nullptr);
+}
+
+int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const lamb_hip_mat* B_,
+                              lamb_hip_mat* C, double alpha, double beta, double eps,
+                              int32_t flags, lamb_hip_stats* stats, const lamb_hip_mat* mask) {
   auto t_start = std::chrono::steady_clock::now();
   if (!ctx || !A_ || !B_ || !C) return LAMB_HIP_FAILURE;
   lamb_hip_mat* A = const_cast<lamb_hip_mat*>(A_);
@@ -131,6 +146,11 @@ extern "C" int32_t lamb_hip_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A_,
                      "multiply: A, B, C must share one distribution");
   if (C == A || C == B)
     return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_WRONG_ARGS, "multiply: C aliases A or B");
+  if (values_missing(A) || values_missing(B) || (beta != 0.0 && values_missing(C)))
+    return refuse_missing_values(ctx, "multiply");
+  if (mask && (mask->dist != d || mask == C))
+    return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_WRONG_ARGS,
+                     "multiply: the mask must share the distribution and not be C");
   const int mode = flags & 0xFF;
   int slices = (flags >> 8) & 0xF;
   if ((flags & ~0xFFF) != 0 || (mode != LAMB_HIP_FP64 && mode != LAMB_HIP_BF16_SPLIT) ||
@@ -213,7 +233,9 @@ extern "C" int32_t lamb_hip_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A_,
                                    false, ncb, ctx->d_bslice.p), "slice B");
   }
   // ---- tiles and entries ----
-  HIPCHK(ctx, build_tiles(s, S.W, D, S.sa, S.sb, use_old ? &MC : nullptr, eps2, S.r),
+  const MatView MM = mask ? mat_view(mask) : MatView();
+  HIPCHK(ctx, build_tiles(s, S.W, D, S.sa, S.sb, use_old ? &MC : nullptr, eps2, S.r,
+                          mask ? &MM : nullptr),
          "tiles");
   if (S.r.n_entries > INT32_MAX || S.r.n_tiles > INT32_MAX)
     return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_OVERFLOW_ERROR,

@@ -46,10 +46,10 @@ struct StripSet {
 
 struct SymScratch {
   DevBuf merged, flag, excl, scan, gslot;
-  DevBuf cnt, ksum, pat, stats, band4, band4s, cells, cellcnt, cellbase;
+  DevBuf cnt, ksum, pat, stats, band4, band4s, cells, cellcnt, cellbase, mpat;
   void release() {
     for (DevBuf* b : {&merged, &flag, &excl, &scan, &gslot, &cnt, &ksum, &pat, &stats, &band4,
-                      &band4s, &cells, &cellcnt, &cellbase})
+                      &band4s, &cells, &cellcnt, &cellbase, &mpat})
       b->release();
   }
 };
@@ -74,8 +74,10 @@ struct CIndex {
 
 hipError_t build_strips(hipStream_t s, SymScratch& W, const MatView& M, const DistView& D,
                         bool a_side, StripSet& out);
+// mask (optional): only blocks in the mask matrix's pattern are produced
 hipError_t build_tiles(hipStream_t s, SymScratch& W, const DistView& D, const StripSet& A,
-                       const StripSet& B, const MatView* Cold, double eps2, SymResult& r);
+                       const StripSet& B, const MatView* Cold, double eps2, SymResult& r,
+                       const MatView* mask = nullptr);
 // perm[k] = tile run k-th by the numeric kernel (blocks of rb row bands,
 // column band by column band)
 hipError_t build_tile_order(hipStream_t s, SymScratch& W, const DistView& D, SymResult& r,

@@ -198,6 +198,47 @@ __device__ __forceinline__ int pair_entries(const Strip* SA, const Strip* SB, do
   return dropped;
 }
 
+// Output mask (masked multiplies): mpat[(I * nbands + J) * 8 + p / 2] holds,
+// in its 16-bit half p % 2, the column atoms q of band J that row atom p of
+// band I may produce. An entry (rm, cm) is cut to the rows with a masked
+// column in cm and to those columns; pairs[p] gets row p's masked columns.
+// Returns false when nothing is left. Without a mask the entry is unchanged.
+__device__ __forceinline__ bool mask_entry(const uint32_t* __restrict__ mpat, int32_t nbands,
+                                           int32_t I, int32_t J, uint32_t& rm, uint32_t& cm,
+                                           uint32_t pairs[kBandAtoms]) {
+  if (!mpat) {
+    for (int p = 0; p < kBandAtoms; ++p) pairs[p] = ((rm >> p) & 1u) ? cm : 0u;
+    return true;
+  }
+  const uint32_t* m = mpat + ((int64_t)I * nbands + J) * (kBandAtoms / 2);
+  uint32_t r2 = 0, c2 = 0;
+  for (int p = 0; p < kBandAtoms; ++p) {
+    uint32_t row = 0;
+    if ((rm >> p) & 1u) row = (m[p >> 1] >> (16 * (p & 1))) & 0xFFFFu & cm;
+    pairs[p] = row;
+    if (row) {
+      r2 |= 1u << p;
+      c2 |= row;
+    }
+  }
+  rm = r2;
+  cm = c2;
+  return r2 != 0;
+}
+
+// mpat bits of every stored block of the mask matrix
+__global__ void mask_pattern(int64_t nblk, const int32_t* __restrict__ rows,
+                             const int32_t* __restrict__ col, const int32_t* __restrict__ band_of,
+                             const int32_t* __restrict__ pos_in_band, int32_t nbands,
+                             uint32_t* __restrict__ mpat) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblk) return;
+  const int32_t i = rows[b], j = col[b];
+  const int32_t I = band_of[i], J = band_of[j], p = pos_in_band[i], q = pos_in_band[j];
+  atomicOr(&mpat[((int64_t)I * nbands + J) * (kBandAtoms / 2) + (p >> 1)],
+           (1u << q) << (16 * (p & 1)));
+}
+
 __global__ __launch_bounds__(64) void sym_count(
     int32_t nbands, int nseg, const Band* __restrict__ bands, const int32_t* __restrict__ blk,
     const Strip* __restrict__ as, const int64_t* __restrict__ a_ptr,
@@ -205,7 +246,7 @@ __global__ __launch_bounds__(64) void sym_count(
     const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
     const int32_t* __restrict__ band_of, const int32_t* __restrict__ pos_in_band,
     int32_t* __restrict__ cnt, int32_t* __restrict__ ksum, uint32_t* __restrict__ pat,
-    unsigned long long* __restrict__ stats) {
+    unsigned long long* __restrict__ stats, const uint32_t* __restrict__ mpat) {
   const int64_t w = blockIdx.x;
   const int32_t I = (int32_t)(w / nseg);
   const int g = (int)(w % nseg);
@@ -232,6 +273,8 @@ __global__ __launch_bounds__(64) void sym_count(
       const int32_t J = SB->band;
       int n = 0;
       dropped += pair_entries(SA, SB, eps2, [&](uint32_t rm, uint32_t cm) {
+        uint32_t pairs[kBandAtoms];
+        if (!mask_entry(mpat, nbands, I, J, rm, cm, pairs)) return;
         // a zero-size K atom adds to the C pattern but holds no entry
         if (kd > 0) ++n;
         kept += (unsigned long long)__builtin_popcount(rm) * __builtin_popcount(cm);
@@ -241,7 +284,7 @@ __global__ __launch_bounds__(64) void sym_count(
           if ((cm >> q) & 1u) cr += bands[J].rs[q + 1] - bands[J].rs[q];
         flops += 2.0 * rows_of_mask(rsI, rm) * kd * cr;
         for (int p = 0; p < kBandAtoms; ++p)
-          if ((rm >> p) & 1u) atomicOr(&pt[(int64_t)J * (kBandAtoms / 2) + (p >> 1)], cm << (16 * (p & 1)));
+          if (pairs[p]) atomicOr(&pt[(int64_t)J * (kBandAtoms / 2) + (p >> 1)], pairs[p] << (16 * (p & 1)));
       });
       if (n) {
         atomicAdd(&c[J], n);
@@ -391,7 +434,7 @@ __global__ __launch_bounds__(64) void sym_fill(
     const int64_t* __restrict__ a_ptr, const Strip* __restrict__ bs,
     const int64_t* __restrict__ b_ptr, const int64_t* __restrict__ a_pk,
     const int64_t* __restrict__ b_pk, double eps2, int32_t* __restrict__ cursor,
-    int32_t* __restrict__ kcur, Entry* __restrict__ entries) {
+    int32_t* __restrict__ kcur, Entry* __restrict__ entries, const uint32_t* __restrict__ mpat) {
   const int64_t w = blockIdx.x;
   const int32_t I = (int32_t)(w / nseg);
   const int g = (int)(w % nseg);
@@ -412,13 +455,18 @@ __global__ __launch_bounds__(64) void sym_fill(
       const int32_t J = SB->band;
       if (kd <= 0) continue;
       int n = 0;
-      pair_entries(SA, SB, eps2, [&](uint32_t, uint32_t) { ++n; });
+      pair_entries(SA, SB, eps2, [&](uint32_t rm, uint32_t cm) {
+        uint32_t pairs[kBandAtoms];
+        if (mask_entry(mpat, nbands, I, J, rm, cm, pairs)) ++n;
+      });
       if (!n) continue;
       // reserve n entry slots and n*kd K columns of tile (I, J); the lanes
       // of one step hold distinct J and steps run in K order
       int32_t e = atomicAdd(&c[J], n);
       int32_t kk = atomicAdd(&kc[J], n * kd);
       pair_entries(SA, SB, eps2, [&](uint32_t rm, uint32_t cm) {
+        uint32_t pairs[kBandAtoms];
+        if (!mask_entry(mpat, nbands, I, J, rm, cm, pairs)) return;
         Entry en;
         en.a_col = (uint32_t)(a_pk[sa] / 64);
         en.b_col = (uint32_t)(b_pk[sb] / 64);
@@ -608,10 +656,23 @@ hipError_t build_strips(hipStream_t s, SymScratch& W, const MatView& M, const Di
 }
 
 hipError_t build_tiles(hipStream_t s, SymScratch& W, const DistView& D, const StripSet& A,
-                       const StripSet& B, const MatView* Cold, double eps2, SymResult& r) {
+                       const StripSet& B, const MatView* Cold, double eps2, SymResult& r,
+                       const MatView* mask) {
   const int32_t nb = D.nbands;
   const int nseg = 8;
   const int64_t dense = (int64_t)nb * nseg * nb;
+  const uint32_t* mp = nullptr;
+  if (mask) {
+    const int64_t words = std::max<int64_t>((int64_t)nb * nb * (kBandAtoms / 2), 1);
+    CK(W.mpat.ensure(sizeof(uint32_t) * (size_t)words));
+    CK(fill32(s, (int32_t*)W.mpat.p, words, 0));
+    if (mask->nblk)
+      hipLaunchKernelGGL(mask_pattern, dim3(grid_of(mask->nblk)), dim3(256), 0, s, mask->nblk,
+                         mask->rows, mask->col, D.band_of, D.pos_in_band, nb,
+                         (uint32_t*)W.mpat.p);
+    CK(hipGetLastError());
+    mp = (const uint32_t*)W.mpat.p;
+  }
   CK(W.cnt.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(dense, 1)));
   CK(W.ksum.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(dense, 1)));
   CK(W.pat.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>((int64_t)nb * nb * (kBandAtoms / 2), 1)));
@@ -626,7 +687,7 @@ hipError_t build_tiles(hipStream_t s, SymScratch& W, const DistView& D, const St
                        (const Strip*)B.strips.p, (const int64_t*)B.ptr.p, eps2,
                        Cold ? Cold->row_p : nullptr, Cold ? Cold->col : nullptr, D.band_of,
                        D.pos_in_band, (int32_t*)W.cnt.p, (int32_t*)W.ksum.p,
-                       (uint32_t*)W.pat.p, (unsigned long long*)W.stats.p);
+                       (uint32_t*)W.pat.p, (unsigned long long*)W.stats.p, mp);
   CK(hipGetLastError());
   // per-band totals and their scans
   CK(W.band4.ensure(sizeof(int64_t) * 4 * (size_t)(nb + 1)));
@@ -668,7 +729,7 @@ hipError_t build_tiles(hipStream_t s, SymScratch& W, const DistView& D, const St
                        (const Strip*)A.strips.p, (const int64_t*)A.ptr.p,
                        (const Strip*)B.strips.p, (const int64_t*)B.ptr.p,
                        (const int64_t*)A.pk.p, (const int64_t*)B.pk.p, eps2, (int32_t*)W.cnt.p,
-                       (int32_t*)W.ksum.p, (Entry*)r.entries.p);
+                       (int32_t*)W.ksum.p, (Entry*)r.entries.p, mp);
   return hipGetLastError();
 }
 
