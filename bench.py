@@ -34,6 +34,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-rows", type=int, default=2,
                    help="CPU baseline samples every n-th block row of A")
+    p.add_argument("--slices", type=int, default=5, help="BF16 slices of the split form")
+    p.add_argument("--no-bf16", action="store_true", help="skip the BF16-split leg")
     return p.parse_args()
 
 
@@ -58,6 +60,39 @@ def cpu_baseline(a, b, eps, every):
         "kind": "port",
         "sample": f"every {every}th block row of A ({int(keep.sum())} of {a.n_atoms} rows), "
                   f"{st['flops']:.3e} flop in {dt:.2f} s, FP64 C oracle, OpenMP",
+    }
+
+
+def bf16_leg(D, da, db, dc, dd, eps, args, ms_step64, ms_num64):
+    """The BF16-split form on the same product: throughput, speed-up over the
+    FP64 path and relative Frobenius error against the FP64 result (dc)."""
+    flags = D.bf16_split(args.slices)
+    c16 = D.DeviceMatrix(dd)
+    for _ in range(max(args.warmup, 1)):
+        st = D.multiply(da, db, c16, eps_filter=eps, flags=flags)
+    sts = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sts.append(D.multiply(da, db, c16, eps_filter=eps, flags=flags))
+    ms_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    ms_num = float(np.mean([s["ms_numeric"] for s in sts]))
+    flops = sts[-1]["flops"]
+    err = float(c16.group_diff_norm(dc)[0] / dc.group_norm()[0])
+    same_pattern = c16.info() == dc.info()
+    c16.close()
+    print(f"bf16 split ({args.slices} slices): {ms_step:.1f} ms/step, numeric {ms_num:.2f} ms, "
+          f"error {err:.2e}", file=sys.stderr, flush=True)
+    return {
+        "slices": args.slices,
+        "products_per_fp64_product": args.slices * (args.slices + 1) // 2,
+        "gflops": flops / (ms_step * 1e-3) / 1e9,
+        "ms_per_step": ms_step,
+        "ms_numeric": ms_num,
+        "speedup_step_vs_fp64": ms_step64 / ms_step,
+        "speedup_numeric_vs_fp64": ms_num64 / ms_num,
+        "rel_frobenius_error_vs_fp64": err,
+        "same_block_count": bool(same_pattern),
+        "mfma_bf16_achieved_tflops": flops * args.slices * (args.slices + 1) / 2 / (ms_num * 1e-3) / 1e12,
     }
 
 
@@ -151,6 +186,8 @@ def main():
             "algorithmic_bytes": bytes_alg,
         },
     }
+    if not args.no_bf16:
+        result["bf16_split"] = bf16_leg(D, da, db, dc, dd, eps, args, ms_step, ms_num)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(a, b, eps, args.cpu_rows)
     # Calibration for hardware byte counters: one device copy of A's data

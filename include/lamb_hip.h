@@ -156,6 +156,15 @@ int32_t lamb_hip_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A, const lamb_h
 /* tr(A): sum of the diagonal elements of the diagonal blocks (FP64). */
 int32_t lamb_hip_trace(const lamb_hip_mat* A, double* trace);
 
+/* Autotuner state (no counterpart in the reference): the FP64 numeric
+ * kernel variant of the context's last multiply (0 quadrant: per-wave
+ * 32 x 32 quadrants; 1 ksplit: whole-tile waves splitting the K steps) and
+ * the number of workload classes tuned so far. The first multiply of a class
+ * (form, largest block, flops and C blocks per tile) times every variant and
+ * keeps the fastest; LAMB_HIP_F64_KERNEL=quadrant|ksplit forces one. */
+int32_t lamb_hip_tuning_info(const lamb_hip_ctx* ctx, int32_t* last_variant,
+                             int32_t* n_classes);
+
 /* --- batches and purification chains --------------------------------------
  * None of these has a counterpart in the reference, whose multiply computes
  * nothing; they are the matrix functions a McWeeny purification chain needs

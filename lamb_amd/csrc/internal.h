@@ -3,7 +3,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -69,6 +71,9 @@ struct lamb_hip_ctx {
   lamb::DevBuf d_aexp, d_bexp;  // BF16 split: exponent per A row / B column
   lamb::DevBuf d_aslice, d_bslice;  // BF16 split: slice packs
   void* sym = nullptr;  // lamb::SymState*
+  // autotuner: numeric kernel variant per workload class (multiply.hip)
+  std::map<std::array<int64_t, 4>, int32_t> tune;
+  int32_t last_variant = -1;
 };
 
 struct lamb_hip_dist {

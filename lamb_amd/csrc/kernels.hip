@@ -125,6 +125,44 @@ constexpr int kTableCols = 2048;
 #endif
 constexpr int kPrefetch = LAMB_NUMERIC_PREFETCH;  // K steps in flight per wave
 
+// The step table of the K window [Q0, Q0 + kTableCols) of a tile: per K
+// column its A and B panel columns (the zero columns past the end) and atom
+// masks, per step of 4 columns the live row / column 4-groups. One thread
+// per entry writes its columns. Returns the window's step count; the table
+// is visible to all threads on return.
+__device__ __forceinline__ int build_step_table(const TileInfo& T, const TileSmem& sm,
+                                                const Entry* __restrict__ entries, int Q0,
+                                                uint2* col_ab, uint32_t* col_m, uint32_t* step_g,
+                                                uint32_t a_zero, uint32_t b_zero) {
+  const int tid = threadIdx.x;
+  const int ncols = min(kTableCols, T.ktot - Q0);
+  const int nsteps = (ncols + 3) >> 2;
+  __syncthreads();  // the previous window is done with the table
+  for (int t = tid; t < 4 * nsteps; t += 256) {
+    col_ab[t] = make_uint2(a_zero, b_zero);
+    col_m[t] = 0u;
+  }
+  for (int t = tid; t < nsteps; t += 256) step_g[t] = 0;
+  __syncthreads();
+  for (int e = tid; e < T.ne; e += 256) {
+    const Entry en = entries[T.e0 + e];
+    const int ks = en.kstart;
+    const int ke = (e + 1 < T.ne) ? entries[T.e0 + e + 1].kstart : T.ktot;
+    const int lo = max(ks, Q0), hi = min(ke, Q0 + ncols);
+    if (lo >= hi) continue;
+    const uint32_t m = (uint32_t)en.rmask | ((uint32_t)en.cmask << 16);
+    const uint32_t g = entry_groups(sm, en.rmask, en.cmask);
+    for (int Q = lo; Q < hi; ++Q) {
+      const int q = Q - Q0;
+      col_ab[q] = make_uint2(en.a_col + (uint32_t)(Q - ks), en.b_col + (uint32_t)(Q - ks));
+      col_m[q] = m;
+    }
+    for (int st = (lo - Q0) >> 2; st <= (hi - 1 - Q0) >> 2; ++st) atomicOr(&step_g[st], g);
+  }
+  __syncthreads();
+  return nsteps;
+}
+
 __global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
     const TileDesc* __restrict__ tiles, const int32_t* __restrict__ perm,
     const Entry* __restrict__ entries, const Band* __restrict__ bands,
@@ -162,32 +200,7 @@ __global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
   for (int x = 0; x < 4; ++x) acc[x] = dbl4{0.0, 0.0, 0.0, 0.0};
 
   for (int Q0 = 0; Q0 < ktot; Q0 += kTableCols) {
-    const int ncols = min(kTableCols, ktot - Q0);
-    const int nsteps = (ncols + 3) >> 2;
-    __syncthreads();  // previous window done
-    for (int t = tid; t < 4 * nsteps; t += 256) {
-      col_ab[t] = make_uint2(a_zero, b_zero);
-      col_m[t] = 0u;
-    }
-    for (int t = tid; t < nsteps; t += 256) step_g[t] = 0;
-    __syncthreads();
-    // one thread per entry writes its columns that fall in this window
-    for (int e = tid; e < ne; e += 256) {
-      const Entry en = entries[e0 + e];
-      const int ks = en.kstart;
-      const int ke = (e + 1 < ne) ? entries[e0 + e + 1].kstart : ktot;
-      const int lo = max(ks, Q0), hi = min(ke, Q0 + ncols);
-      if (lo >= hi) continue;
-      const uint32_t m = (uint32_t)en.rmask | ((uint32_t)en.cmask << 16);
-      const uint32_t g = entry_groups(sm, en.rmask, en.cmask);
-      for (int Q = lo; Q < hi; ++Q) {
-        const int q = Q - Q0;
-        col_ab[q] = make_uint2(en.a_col + (uint32_t)(Q - ks), en.b_col + (uint32_t)(Q - ks));
-        col_m[q] = m;
-      }
-      for (int st = (lo - Q0) >> 2; st <= (hi - 1 - Q0) >> 2; ++st) atomicOr(&step_g[st], g);
-    }
-    __syncthreads();
+    const int nsteps = build_step_table(T, sm, entries, Q0, col_ab, col_m, step_g, a_zero, b_zero);
 
     // ---- per-wave walk over the window's live steps, kPrefetch ahead ----
     int cbase = -64;        // first step of the current chunk
@@ -274,17 +287,186 @@ __global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
                 kept_out);
 }
 
+// ---------------------------------------------------------------------------
+// FP64 numeric phase, K-split variant. One workgroup (4 waves) per C tile as
+// above, but every wave owns the whole 64 x 64 tile (sixteen 16 x 16
+// accumulators) and the waves split the tile's live K steps: wave w takes
+// the live steps whose rank is w mod 4. A step then feeds up to 16 MFMAs from
+// 8 loads (the quadrant kernel: 4 MFMAs from 4 loads, each operand loaded by
+// two waves). The four partial tiles are summed in LDS in wave order, so the
+// result is deterministic. Occupancy: two workgroups per CU.
+// ---------------------------------------------------------------------------
+#ifndef LAMB_KSPLIT_PREFETCH
+#define LAMB_KSPLIT_PREFETCH 2
+#endif
+constexpr int kKsPrefetch = LAMB_KSPLIT_PREFETCH;
+
+// live 16 x 16 blocks of the whole tile in a step with row / column groups g:
+// bit 4h + q = row block h and column block q both hold a live 4-group
+__device__ __forceinline__ uint32_t tile_blocks(uint32_t g) {
+  const uint32_t rg = g & 0xFFFFu, cg = g >> 16;
+  uint32_t rh = 0, cq = 0;
+#pragma unroll
+  for (int h = 0; h < 4; ++h) {
+    rh |= ((rg >> (4 * h)) & 0xFu) ? 1u << h : 0u;
+    cq |= ((cg >> (4 * h)) & 0xFu) ? 1u << h : 0u;
+  }
+  uint32_t bm = 0;
+#pragma unroll
+  for (int h = 0; h < 4; ++h) bm |= ((rh >> h) & 1u) ? cq << (4 * h) : 0u;
+  return bm;
+}
+
+__global__ __launch_bounds__(256, 2) void numeric_f64_ksplit_kernel(
+    const TileDesc* __restrict__ tiles, const int32_t* __restrict__ perm,
+    const Entry* __restrict__ entries, const Band* __restrict__ bands,
+    const double* __restrict__ a_pack, const double* __restrict__ b_pack, uint32_t a_zero,
+    uint32_t b_zero, double alpha, double beta, double eps2,
+    const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
+    const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
+    double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
+  __shared__ double lds[kBandRows * kPitch];
+  uint2* col_ab = reinterpret_cast<uint2*>(lds);
+  uint32_t* col_m = reinterpret_cast<uint32_t*>(col_ab + kTableCols);
+  uint32_t* step_g = col_m + kTableCols;
+  __shared__ TileSmem sm;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const TileInfo T = tile_setup(tiles, perm, bands, sm);
+  // atom mask bit of this lane's row in row block h / column in column block q
+  uint32_t rbit[4] = {0u, 0u, 0u, 0u}, cbit[4] = {0u, 0u, 0u, 0u};
+  for (int a = 0; a < T.natI; ++a)
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      const int r = 16 * h + (lane & 15);
+      if (r >= sm.rs_r[a] && r < sm.rs_r[a + 1]) rbit[h] = 1u << a;
+    }
+  for (int a = 0; a < T.natJ; ++a)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = 16 * q + (lane & 15);
+      if (c >= sm.rs_c[a] && c < sm.rs_c[a + 1]) cbit[q] = 1u << (16 + a);
+    }
+  const int kc = lane >> 4;
+  const double* a_lane = a_pack + (lane & 15);
+  const double* b_lane = b_pack + (lane & 15);
+
+  dbl4 acc[16];
+#pragma unroll
+  for (int x = 0; x < 16; ++x) acc[x] = dbl4{0.0, 0.0, 0.0, 0.0};
+  uint32_t seen = 0;  // live steps met so far (wave-uniform)
+
+  for (int Q0 = 0; Q0 < T.ktot; Q0 += kTableCols) {
+    const int nsteps = build_step_table(T, sm, entries, Q0, col_ab, col_m, step_g, a_zero, b_zero);
+    int cbase = -64;
+    uint64_t live = 0;
+    uint32_t chunk_bm = 0;
+    double ra[8][4], rb[8][4];
+    uint32_t blkmask[8];
+#define KS_FETCH(S)                                                                \
+    {                                                                              \
+      uint32_t bm = 0;                                                             \
+      int st = 0;                                                                  \
+      for (;;) {                                                                   \
+        while (live == 0 && cbase + 64 < nsteps) {                                 \
+          cbase += 64;                                                             \
+          const int sl = cbase + lane;                                             \
+          chunk_bm = tile_blocks(sl < nsteps ? step_g[sl] : 0u);                   \
+          live = __ballot(chunk_bm != 0u);                                         \
+        }                                                                          \
+        if (live == 0) break;                                                      \
+        const int l = __builtin_ctzll(live);                                       \
+        live &= live - 1;                                                          \
+        if ((seen++ & 3u) == (uint32_t)wave) {                                     \
+          st = cbase + l;                                                          \
+          bm = __builtin_amdgcn_readlane(chunk_bm, l);                             \
+          break;                                                                   \
+        }                                                                          \
+      }                                                                            \
+      blkmask[S] = bm;                                                             \
+      const int Q = 4 * st + kc;                                                   \
+      const uint2 ab = col_ab[Q];                                                  \
+      const uint32_t cm = col_m[Q];                                                \
+      _Pragma("unroll") for (int h = 0; h < 4; ++h) {                              \
+        const uint32_t need = (bm & (0xFu << (4 * h))) ? rbit[h] : 0u;             \
+        const uint32_t c = (cm & need) ? ab.x : a_zero;                            \
+        ra[S][h] = a_lane[(int64_t)c * 64 + 16 * h];                               \
+      }                                                                            \
+      _Pragma("unroll") for (int q = 0; q < 4; ++q) {                              \
+        const uint32_t need = (bm & (0x1111u << q)) ? cbit[q] : 0u;                \
+        const uint32_t c = (cm & need) ? ab.y : b_zero;                            \
+        rb[S][q] = b_lane[(int64_t)c * 64 + 16 * q];                               \
+      }                                                                            \
+    }
+#define KS_MFMA(S)                                                                 \
+    {                                                                              \
+      const uint32_t bm = blkmask[S];                                              \
+      _Pragma("unroll") for (int h = 0; h < 4; ++h)                                \
+        _Pragma("unroll") for (int q = 0; q < 4; ++q)                              \
+          if ((bm >> (4 * h + q)) & 1u)                                            \
+            acc[4 * h + q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[S][h], rb[S][q], acc[4 * h + q], 0, 0, 0); \
+    }
+#define KS_SLOTS(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
+    static_assert(kKsPrefetch >= 1 && kKsPrefetch <= 8, "prefetch depth 1..8");
+#define KS_PRIME(S) if (S < kKsPrefetch) KS_FETCH(S)
+#define KS_ANY(S) if (S < kKsPrefetch) any |= blkmask[S];
+#define KS_STEP(S) if (S < kKsPrefetch) { KS_MFMA(S); KS_FETCH(S); }
+    KS_SLOTS(KS_PRIME)
+    for (;;) {
+      uint32_t any = 0;
+      KS_SLOTS(KS_ANY)
+      if (!any) break;
+      KS_SLOTS(KS_STEP)
+    }
+#undef KS_STEP
+#undef KS_ANY
+#undef KS_PRIME
+#undef KS_SLOTS
+#undef KS_FETCH
+#undef KS_MFMA
+  }
+  __syncthreads();
+
+  // ---- epilogue: the four partial tiles summed into LDS in wave order ----
+  // v_mfma_f64_16x16x4 result: D[i][j] in lane 16*(i%4) + j, register i/4.
+  double* Ct = lds;
+  for (int w = 0; w < 4; ++w) {
+    if (wave == w) {
+#pragma unroll
+      for (int h = 0; h < 4; ++h)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int idx = (16 * h + 4 * r + (lane >> 4)) * kPitch + 16 * q + (lane & 15);
+            const double v = acc[4 * h + q][r];
+            Ct[idx] = w == 0 ? v : Ct[idx] + v;
+          }
+    }
+    __syncthreads();
+  }
+  tile_epilogue(Ct, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data, c_data,
+                kept_out);
+}
+
 hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* tiles,
                               const int32_t* perm, const Entry* entries, const Band* bands,
                               const double* a_pack, const double* b_pack, uint32_t a_zero,
-                              uint32_t b_zero, double alpha,
-                              double beta, double eps2, const int64_t* old_row_p,
-                              const int32_t* old_col, const int64_t* old_off,
-                              const double* old_data, double* c_data, uint16_t* kept) {
+                              uint32_t b_zero, double alpha, double beta, double eps2,
+                              const int64_t* old_row_p, const int32_t* old_col,
+                              const int64_t* old_off, const double* old_data, double* c_data,
+                              uint16_t* kept, int variant) {
   if (n_tiles == 0) return hipSuccess;
-  hipLaunchKernelGGL(numeric_f64_kernel, dim3((unsigned)n_tiles), dim3(256), 0, s, tiles, perm,
-                     entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta, eps2, old_row_p, old_col,
-                     old_off, old_data, c_data, kept);
+  if (variant == kF64Ksplit)
+    hipLaunchKernelGGL(numeric_f64_ksplit_kernel, dim3((unsigned)n_tiles), dim3(256), 0, s, tiles,
+                       perm, entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta, eps2,
+                       old_row_p, old_col, old_off, old_data, c_data, kept);
+  else
+    hipLaunchKernelGGL(numeric_f64_kernel, dim3((unsigned)n_tiles), dim3(256), 0, s, tiles, perm,
+                       entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta, eps2,
+                       old_row_p, old_col, old_off, old_data, c_data, kept);
   return hipGetLastError();
 }
 

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <array>
 #include <chrono>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
@@ -103,6 +104,18 @@ int32_t lamb::refuse_missing_values(lamb_hip_ctx* ctx, const char* what) {
   return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_PRECONDITION_FAILED,
                    "%s: the matrix's values were released", what);
 }
+
+// FP64 kernel variant forced by LAMB_HIP_F64_KERNEL (quadrant | ksplit), or
+// -1: chosen by the autotuner
+static int f64_forced() {
+  const char* v = getenv("LAMB_HIP_F64_KERNEL");
+  if (v && strcmp(v, "quadrant") == 0) return kF64Quadrant;
+  if (v && strcmp(v, "ksplit") == 0) return kF64Ksplit;
+  return -1;
+}
+
+// half-octave bucket of a positive size
+static int64_t bucket(double x) { return (int64_t)std::floor(2.0 * std::log2(std::max(x, 1.0))); }
 
 static DistView dist_view(const lamb_hip_dist* d) {
   DistView v;
@@ -255,18 +268,54 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
   DevBuf newdata;
   HIPCHK(ctx, newdata.ensure(sizeof(double) * (size_t)std::max<int64_t>(S.r.c_size, 1)),
          "C pool");
+  auto launch_f64 = [&](int variant) {
+    return launch_numeric_f64(s, n_tiles, (const TileDesc*)S.r.tiles.p,
+                              (const int32_t*)S.r.perm.p, (const Entry*)S.r.entries.p, D.bands,
+                              (const double*)ctx->d_apack.p, (const double*)ctx->d_bpack.p,
+                              a_zero, b_zero, alpha, beta, eps2,
+                              use_old ? MC.row_p : nullptr, use_old ? MC.col : nullptr,
+                              use_old ? MC.off : nullptr,
+                              use_old ? (const double*)C->d_data.p : nullptr,
+                              (double*)newdata.p, (uint16_t*)ctx->d_kept.p, variant);
+  };
+  int variant = 0;
+  if (!split) {
+    // Autotuner: the FP64 variants trade occupancy (quadrant) against work
+    // per K step (ksplit); which wins depends on how dense the tiles are. A
+    // workload class is (form, largest block, flops per tile, C blocks per
+    // tile) in half-octave buckets; the first multiply of a class times every
+    // variant on its own tiles (the launches only rewrite this call's C pool)
+    // and the winner is kept for the context's lifetime.
+    variant = f64_forced();
+    if (variant < 0) {
+      const double nt = std::max<double>(1.0, (double)n_tiles);
+      const std::array<int64_t, 4> key = {flags, d->max_blk, bucket(S.r.flops / nt),
+                                           bucket((double)S.r.n_struct / nt)};
+      auto it = ctx->tune.find(key);
+      if (it != ctx->tune.end()) {
+        variant = it->second;
+      } else {
+        float best = 0.f;
+        for (int v : {kF64Quadrant, kF64Ksplit}) {
+          HIPCHK(ctx, hipEventRecord(ctx->ev[5], s), "event");
+          HIPCHK(ctx, launch_f64(v), "numeric kernel (tuning)");
+          HIPCHK(ctx, hipEventRecord(ctx->ev[6], s), "event");
+          HIPCHK(ctx, hipEventSynchronize(ctx->ev[6]), "tuning sync");
+          float ms = 0.f;
+          HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[5], ctx->ev[6]), "event time");
+          if (v == kF64Quadrant || ms < best) {
+            best = ms;
+            variant = v;
+          }
+        }
+        ctx->tune[key] = variant;
+      }
+    }
+    ctx->last_variant = variant;
+  }
   HIPCHK(ctx, hipEventRecord(ctx->ev[0], s), "event");
   if (!split)
-    HIPCHK(ctx, launch_numeric_f64(s, n_tiles, (const TileDesc*)S.r.tiles.p,
-                                   (const int32_t*)S.r.perm.p,
-                                   (const Entry*)S.r.entries.p, D.bands,
-                                   (const double*)ctx->d_apack.p, (const double*)ctx->d_bpack.p,
-                                   a_zero, b_zero, alpha, beta, eps2,
-                                   use_old ? MC.row_p : nullptr, use_old ? MC.col : nullptr,
-                                   use_old ? MC.off : nullptr,
-                                   use_old ? (const double*)C->d_data.p : nullptr,
-                                   (double*)newdata.p, (uint16_t*)ctx->d_kept.p),
-           "numeric kernel");
+    HIPCHK(ctx, launch_f64(variant), "numeric kernel");
   else
     HIPCHK(ctx, launch_numeric_bf16(s, slices, n_tiles, (const TileDesc*)S.r.tiles.p,
                                     (const int32_t*)S.r.perm.p, (const Entry*)S.r.entries.p,
@@ -325,5 +374,13 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
     stats->ms_total = std::chrono::duration<double, std::milli>(
                           std::chrono::steady_clock::now() - t_start).count();
   }
+  return LAMB_HIP_OK;
+}
+
+extern "C" int32_t lamb_hip_tuning_info(const lamb_hip_ctx* ctx, int32_t* last_variant,
+                                        int32_t* n_classes) {
+  if (!ctx) return LAMB_HIP_FAILURE;
+  if (last_variant) *last_variant = ctx->last_variant;
+  if (n_classes) *n_classes = (int32_t)ctx->tune.size();
   return LAMB_HIP_OK;
 }

@@ -189,6 +189,13 @@ class DeviceMatrix:
             self.h = None
 
 
+def tuning_info(ctx: Context):
+    """(FP64 kernel variant of the last multiply, workload classes tuned)."""
+    v, n = C.c_int32(), C.c_int32()
+    ctx.check(ctx.lib.lamb_hip_tuning_info(ctx.h, C.byref(v), C.byref(n)), "tuning_info")
+    return v.value, n.value
+
+
 def purification_start(h: DeviceMatrix) -> DeviceMatrix:
     """P0 = (H - lo I) / (hi - lo) per group, lo / hi the Gershgorin bounds."""
     lo, hi = h.group_gershgorin()
