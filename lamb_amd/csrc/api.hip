@@ -268,6 +268,15 @@ void lamb_hip_mat::release_device() {
   d_off.release();
   d_rows.release();
   d_nsq.release();
+  release_spares();
+}
+
+void lamb_hip_mat::release_spares() {
+  s_data.release();
+  s_row_p.release();
+  s_col.release();
+  s_off.release();
+  s_rows.release();
 }
 
 void lamb_hip_mat::invalidate_derived() {
@@ -310,7 +319,7 @@ int32_t lamb_hip_matrix_copy(const lamb_hip_mat* src, lamb_hip_mat* dst) {
   dst->invalidate_derived();
   dst->staged_sectors.clear();
   dst->staged_lists.clear();
-  HIPCHK(ctx, hipStreamSynchronize(s), "copy sync");
+  HIPCHK(ctx, stream_sync(ctx), "copy sync");
   return LAMB_HIP_OK;
 }
 

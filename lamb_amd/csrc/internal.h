@@ -74,7 +74,23 @@ struct lamb_hip_ctx {
   // autotuner: numeric kernel variant per workload class (multiply.hip)
   std::map<std::array<int64_t, 4>, int32_t> tune;
   int32_t last_variant = -1;
+  // Sizes the host reads back from the device (graph.hip). kSizesLive: read
+  // them (stream sync); kSizesRecord: read and append to size_log;
+  // kSizesReplay (during a graph capture): take them from size_log in order
+  // and enqueue a check that raises d_guard when the device value differs.
+  int32_t size_mode = 0;
+  std::vector<int64_t> size_log;
+  size_t size_cursor = 0;
+  lamb::DevBuf d_guard;  // int64: a replayed size differed from the recorded one
 };
+
+namespace lamb {
+constexpr int32_t kSizesLive = 0, kSizesRecord = 1, kSizesReplay = 2;
+// n int64 sizes at device address src into dst, according to ctx->size_mode
+hipError_t read_sizes(lamb_hip_ctx* ctx, const void* src, int64_t* dst, int n);
+// Waits for the context stream, except while a graph is being captured.
+hipError_t stream_sync(lamb_hip_ctx* ctx);
+}  // namespace lamb
 
 struct lamb_hip_dist {
   lamb_hip_ctx* ctx = nullptr;
@@ -115,6 +131,9 @@ struct lamb_hip_mat {
   int64_t ndata = 0;   // size of the data pool (may hold unused gaps)
   int64_t nelems = 0;  // values in stored blocks
   lamb::DevBuf d_data, d_row_p, d_col, d_off, d_rows, d_nsq;
+  // spare data / index buffers a multiply writes C into before swapping
+  // (kept only while a graph is recorded or captured)
+  lamb::DevBuf s_data, s_row_p, s_col, s_off, s_rows;
   std::vector<double> nsq;  // host copy of the block norms
   bool nsq_valid = false;
   std::vector<double> host_data;
@@ -125,5 +144,6 @@ struct lamb_hip_mat {
   hipError_t upload_index(hipStream_t s);
   hipError_t download_index(hipStream_t s);
   void release_device();
+  void release_spares();
   void invalidate_derived();
 };

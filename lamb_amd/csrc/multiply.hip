@@ -105,6 +105,12 @@ int32_t lamb::refuse_missing_values(lamb_hip_ctx* ctx, const char* what) {
                    "%s: the matrix's values were released", what);
 }
 
+// phase timing events (not recorded while a graph is being captured)
+static hipError_t record_event(lamb_hip_ctx* ctx, int k) {
+  if (ctx->size_mode == kSizesReplay) return hipSuccess;
+  return hipEventRecord(ctx->ev[k], ctx->stream);
+}
+
 // FP64 kernel variant forced by LAMB_HIP_F64_KERNEL (quadrant | ksplit), or
 // -1: chosen by the autotuner
 static int f64_forced() {
@@ -191,10 +197,10 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
   const DistView D = dist_view(d);
   const MatView MA = mat_view(A), MB = mat_view(B), MC = mat_view(C);
 
-  HIPCHK(ctx, hipEventRecord(ctx->ev[2], s), "event");
+  HIPCHK(ctx, record_event(ctx, 2), "event");
   // ---- strips and packed panels ----
-  HIPCHK(ctx, build_strips(s, S.W, MA, D, true, S.sa), "A strips");
-  HIPCHK(ctx, build_strips(s, S.W, MB, D, false, S.sb), "B strips");
+  HIPCHK(ctx, build_strips(ctx, S.W, MA, D, true, S.sa), "A strips");
+  HIPCHK(ctx, build_strips(ctx, S.W, MB, D, false, S.sb), "B strips");
   // panel columns are addressed by 32-bit indices; one zero column follows
   // the last panel of each side
   if (S.sa.pk_total / 64 >= (int64_t)UINT32_MAX || S.sb.pk_total / 64 >= (int64_t)UINT32_MAX ||
@@ -247,7 +253,7 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
   }
   // ---- tiles and entries ----
   const MatView MM = mask ? mat_view(mask) : MatView();
-  HIPCHK(ctx, build_tiles(s, S.W, D, S.sa, S.sb, use_old ? &MC : nullptr, eps2, S.r,
+  HIPCHK(ctx, build_tiles(ctx, S.W, D, S.sa, S.sb, use_old ? &MC : nullptr, eps2, S.r,
                           mask ? &MM : nullptr),
          "tiles");
   if (S.r.n_entries > INT32_MAX || S.r.n_tiles > INT32_MAX)
@@ -257,15 +263,18 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
     // tiles run in blocks of rb row bands (tuning knob, default 16)
     const char* env = getenv("LAMB_HIP_TILE_ROWBLOCK");
     const int rb = env ? atoi(env) : 16;
-    HIPCHK(ctx, build_tile_order(s, S.W, D, S.r, std::max(1, std::min(rb, 32))), "tile order");
+    HIPCHK(ctx, build_tile_order(ctx, S.W, D, S.r, std::max(1, std::min(rb, 32))), "tile order");
   }
-  HIPCHK(ctx, hipEventRecord(ctx->ev[3], s), "event");
+  HIPCHK(ctx, record_event(ctx, 3), "event");
 
   // ---- numeric ----
   const int32_t n_tiles = (int32_t)S.r.n_tiles;
   HIPCHK(ctx, ctx->d_kept.ensure(sizeof(uint16_t) * kBandAtoms * (size_t)std::max(n_tiles, 1)),
          "kept");
-  DevBuf newdata;
+  // the new C goes into C's spare buffers, swapped in at the end (a graph
+  // capture allocates and frees nothing: the spares are grown beforehand)
+  DevBuf newdata = C->s_data;
+  C->s_data = DevBuf();
   HIPCHK(ctx, newdata.ensure(sizeof(double) * (size_t)std::max<int64_t>(S.r.c_size, 1)),
          "C pool");
   auto launch_f64 = [&](int variant) {
@@ -294,12 +303,14 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
       auto it = ctx->tune.find(key);
       if (it != ctx->tune.end()) {
         variant = it->second;
+      } else if (ctx->size_mode == kSizesReplay) {
+        variant = kF64Quadrant;  // no timing while capturing (the recorded run tuned)
       } else {
         float best = 0.f;
         for (int v : {kF64Quadrant, kF64Ksplit}) {
-          HIPCHK(ctx, hipEventRecord(ctx->ev[5], s), "event");
+          HIPCHK(ctx, record_event(ctx, 5), "event");
           HIPCHK(ctx, launch_f64(v), "numeric kernel (tuning)");
-          HIPCHK(ctx, hipEventRecord(ctx->ev[6], s), "event");
+          HIPCHK(ctx, record_event(ctx, 6), "event");
           HIPCHK(ctx, hipEventSynchronize(ctx->ev[6]), "tuning sync");
           float ms = 0.f;
           HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[5], ctx->ev[6]), "event time");
@@ -313,7 +324,7 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
     }
     ctx->last_variant = variant;
   }
-  HIPCHK(ctx, hipEventRecord(ctx->ev[0], s), "event");
+  HIPCHK(ctx, record_event(ctx, 0), "event");
   if (!split)
     HIPCHK(ctx, launch_f64(variant), "numeric kernel");
   else
@@ -325,29 +336,38 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
                                     use_old ? (const double*)C->d_data.p : nullptr,
                                     (double*)newdata.p, (uint16_t*)ctx->d_kept.p),
            "numeric kernel (BF16 split)");
-  HIPCHK(ctx, hipEventRecord(ctx->ev[1], s), "event");
+  HIPCHK(ctx, record_event(ctx, 1), "event");
 
   // ---- C index ----
   CIndex ci;
-  HIPCHK(ctx, build_c_index(s, S.W, D, S.r, (const uint16_t*)ctx->d_kept.p, ci), "C index");
-  HIPCHK(ctx, hipEventRecord(ctx->ev[4], s), "event");
-  HIPCHK(ctx, hipStreamSynchronize(s), "multiply sync");
+  ci.row_p = C->s_row_p;
+  ci.col = C->s_col;
+  ci.off = C->s_off;
+  ci.rows = C->s_rows;
+  C->s_row_p = C->s_col = C->s_off = C->s_rows = DevBuf();
+  HIPCHK(ctx, build_c_index(ctx, S.W, D, S.r, (const uint16_t*)ctx->d_kept.p, ci), "C index");
+  HIPCHK(ctx, record_event(ctx, 4), "event");
+  HIPCHK(ctx, stream_sync(ctx), "multiply sync");
   float ms_num = 0.f, ms_sym = 0.f, ms_idx = 0.f;
-  HIPCHK(ctx, hipEventElapsedTime(&ms_num, ctx->ev[0], ctx->ev[1]), "event time");
-  HIPCHK(ctx, hipEventElapsedTime(&ms_sym, ctx->ev[2], ctx->ev[3]), "event time");
-  HIPCHK(ctx, hipEventElapsedTime(&ms_idx, ctx->ev[1], ctx->ev[4]), "event time");
+  if (ctx->size_mode != kSizesReplay) {
+    HIPCHK(ctx, hipEventElapsedTime(&ms_num, ctx->ev[0], ctx->ev[1]), "event time");
+    HIPCHK(ctx, hipEventElapsedTime(&ms_sym, ctx->ev[2], ctx->ev[3]), "event time");
+    HIPCHK(ctx, hipEventElapsedTime(&ms_idx, ctx->ev[1], ctx->ev[4]), "event time");
+  }
 
-  // swap in the new C
-  C->d_data.release();
-  C->d_data = newdata;
-  C->d_row_p.release();
-  C->d_row_p = ci.row_p;
-  C->d_col.release();
-  C->d_col = ci.col;
-  C->d_off.release();
-  C->d_off = ci.off;
-  C->d_rows.release();
-  C->d_rows = ci.rows;
+  // swap in the new C; the old buffers become the spares (freed unless a
+  // graph is being recorded or captured)
+  std::swap(C->d_data, newdata);
+  C->s_data = newdata;
+  std::swap(C->d_row_p, ci.row_p);
+  std::swap(C->d_col, ci.col);
+  std::swap(C->d_off, ci.off);
+  std::swap(C->d_rows, ci.rows);
+  C->s_row_p = ci.row_p;
+  C->s_col = ci.col;
+  C->s_off = ci.off;
+  C->s_rows = ci.rows;
+  if (ctx->size_mode == kSizesLive) C->release_spares();
   C->nblk = ci.nblk;
   C->ndata = S.r.c_size;
   C->nelems = ci.nelems;

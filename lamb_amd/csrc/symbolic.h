@@ -72,17 +72,17 @@ struct CIndex {
   DevBuf row_p, col, off, rows;
 };
 
-hipError_t build_strips(hipStream_t s, SymScratch& W, const MatView& M, const DistView& D,
+hipError_t build_strips(lamb_hip_ctx* ctx, SymScratch& W, const MatView& M, const DistView& D,
                         bool a_side, StripSet& out);
 // mask (optional): only blocks in the mask matrix's pattern are produced
-hipError_t build_tiles(hipStream_t s, SymScratch& W, const DistView& D, const StripSet& A,
+hipError_t build_tiles(lamb_hip_ctx* ctx, SymScratch& W, const DistView& D, const StripSet& A,
                        const StripSet& B, const MatView* Cold, double eps2, SymResult& r,
                        const MatView* mask = nullptr);
 // perm[k] = tile run k-th by the numeric kernel (blocks of rb row bands,
 // column band by column band)
-hipError_t build_tile_order(hipStream_t s, SymScratch& W, const DistView& D, SymResult& r,
+hipError_t build_tile_order(lamb_hip_ctx* ctx, SymScratch& W, const DistView& D, SymResult& r,
                             int rb);
-hipError_t build_c_index(hipStream_t s, SymScratch& W, const DistView& D, const SymResult& r,
-                         const uint16_t* kept, CIndex& c);
+hipError_t build_c_index(lamb_hip_ctx* ctx, SymScratch& W, const DistView& D,
+                         const SymResult& r, const uint16_t* kept, CIndex& c);
 
 }  // namespace lamb

@@ -580,6 +580,15 @@ __global__ void tile_perm(int64_t ntiles, int rb, int32_t nbands, const TileDesc
   perm[base[c] + __builtin_popcount(below)] = (int32_t)t;
 }
 
+// out[0..3] = the four band-total sums (after their scans, at [v*(nb+2)+nb]),
+// out[4..6] = kept, dropped, flops statistics
+__global__ void gather_totals(const int64_t* __restrict__ bts, int32_t nb,
+                              const int64_t* __restrict__ stats, int64_t* __restrict__ out) {
+  const int t = threadIdx.x;
+  if (t < 4) out[t] = bts[(int64_t)t * (nb + 2) + nb];
+  else if (t < 7) out[t] = stats[t - 4];
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -591,8 +600,9 @@ __global__ void tile_perm(int64_t ntiles, int rb, int32_t nbands, const TileDesc
 
 static unsigned grid_of(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
-hipError_t build_strips(hipStream_t s, SymScratch& W, const MatView& M, const DistView& D,
+hipError_t build_strips(lamb_hip_ctx* ctx, SymScratch& W, const MatView& M, const DistView& D,
                         bool a_side, StripSet& out) {
+  hipStream_t s = ctx->stream;
   const int64_t nblk = M.nblk;
   // the slot sequence: merged (A) or CSR (B) order
   CK(W.flag.ensure(sizeof(int64_t) * (size_t)(nblk + 1)));
@@ -611,8 +621,7 @@ hipError_t build_strips(hipStream_t s, SymScratch& W, const MatView& M, const Di
   CK(hipGetLastError());
   CK(exclusive_scan(s, (const int64_t*)W.flag.p, (int64_t*)W.excl.p, nblk, (int64_t*)W.scan.p));
   int64_t nstrips = 0;
-  CK(hipMemcpyAsync(&nstrips, (int64_t*)W.excl.p + nblk, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  CK(hipStreamSynchronize(s));
+  CK(read_sizes(ctx, (int64_t*)W.excl.p + nblk, &nstrips, 1));
   out.n = nstrips;
   CK(out.strips.ensure(sizeof(Strip) * (size_t)std::max<int64_t>(nstrips, 1)));
   if (nstrips)
@@ -649,15 +658,14 @@ hipError_t build_strips(hipStream_t s, SymScratch& W, const MatView& M, const Di
     hipLaunchKernelGGL(strip_panel_size, dim3(grid_of(nstrips)), dim3(256), 0, s, nstrips,
                        (const Strip*)out.strips.p, D.blk, (int64_t*)W.flag.p);
   CK(exclusive_scan(s, (const int64_t*)W.flag.p, (int64_t*)out.pk.p, nstrips, (int64_t*)W.scan.p));
-  CK(hipMemcpyAsync(&out.pk_total, (int64_t*)out.pk.p + nstrips, sizeof(int64_t),
-                    hipMemcpyDeviceToHost, s));
-  CK(hipStreamSynchronize(s));
+  CK(read_sizes(ctx, (int64_t*)out.pk.p + nstrips, &out.pk_total, 1));
   return hipSuccess;
 }
 
-hipError_t build_tiles(hipStream_t s, SymScratch& W, const DistView& D, const StripSet& A,
+hipError_t build_tiles(lamb_hip_ctx* ctx, SymScratch& W, const DistView& D, const StripSet& A,
                        const StripSet& B, const MatView* Cold, double eps2, SymResult& r,
                        const MatView* mask) {
+  hipStream_t s = ctx->stream;
   const int32_t nb = D.nbands;
   const int nseg = 8;
   const int64_t dense = (int64_t)nb * nseg * nb;
@@ -676,7 +684,7 @@ hipError_t build_tiles(hipStream_t s, SymScratch& W, const DistView& D, const St
   CK(W.cnt.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(dense, 1)));
   CK(W.ksum.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(dense, 1)));
   CK(W.pat.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>((int64_t)nb * nb * (kBandAtoms / 2), 1)));
-  CK(W.stats.ensure(sizeof(unsigned long long) * 4));
+  CK(W.stats.ensure(sizeof(unsigned long long) * 16));
   CK(fill32(s, (int32_t*)W.cnt.p, dense, 0));
   CK(fill32(s, (int32_t*)W.ksum.p, dense, 0));
   CK(fill32(s, (int32_t*)W.pat.p, (int64_t)nb * nb * (kBandAtoms / 2), 0));
@@ -702,19 +710,20 @@ hipError_t build_tiles(hipStream_t s, SymScratch& W, const DistView& D, const St
   CK(W.scan.ensure(sizeof(int64_t) * (size_t)scan_scratch_elems(nb + 1)));
   for (int v = 0; v < 4; ++v)
     CK(exclusive_scan(s, bt + v * (nb + 1), bts + v * (nb + 2), nb, (int64_t*)W.scan.p));
-  int64_t tot[4];
-  unsigned long long st[3];
-  for (int v = 0; v < 4; ++v)
-    CK(hipMemcpyAsync(&tot[v], bts + v * (nb + 2) + nb, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  CK(hipMemcpyAsync(st, W.stats.p, sizeof(st), hipMemcpyDeviceToHost, s));
-  CK(hipStreamSynchronize(s));
+  // the four totals and three statistics, gathered for one read
+  int64_t* gathered = (int64_t*)W.stats.p + 8;
+  hipLaunchKernelGGL(gather_totals, dim3(1), dim3(64), 0, s, bts, nb,
+                     (const int64_t*)W.stats.p, gathered);
+  CK(hipGetLastError());
+  int64_t tot[7];
+  CK(read_sizes(ctx, gathered, tot, 7));
   r.n_tiles = tot[0];
   r.n_entries = tot[1];
   r.c_size = tot[2];
   r.n_struct = tot[3];
-  r.kept = (int64_t)st[0];
-  r.dropped = (int64_t)st[1];
-  r.flops = (double)st[2];
+  r.kept = tot[4];
+  r.dropped = tot[5];
+  r.flops = (double)(unsigned long long)tot[6];
   CK(r.tiles.ensure(sizeof(TileDesc) * (size_t)std::max<int64_t>(r.n_tiles, 1)));
   CK(r.entries.ensure(sizeof(Entry) * (size_t)std::max<int64_t>(r.n_entries, 1)));
   CK(r.tile_base.ensure(sizeof(int64_t) * (size_t)(nb + 1)));
@@ -733,8 +742,9 @@ hipError_t build_tiles(hipStream_t s, SymScratch& W, const DistView& D, const St
   return hipGetLastError();
 }
 
-hipError_t build_c_index(hipStream_t s, SymScratch& W, const DistView& D, const SymResult& r,
-                         const uint16_t* kept, CIndex& c) {
+hipError_t build_c_index(lamb_hip_ctx* ctx, SymScratch& W, const DistView& D,
+                         const SymResult& r, const uint16_t* kept, CIndex& c) {
+  hipStream_t s = ctx->stream;
   const int32_t nb = D.nbands;
   CK(c.row_p.ensure(sizeof(int64_t) * (size_t)(D.n + 1)));
   CK(W.flag.ensure(sizeof(int64_t) * (size_t)(D.n + 1)));
@@ -746,8 +756,7 @@ hipError_t build_c_index(hipStream_t s, SymScratch& W, const DistView& D, const 
   CK(hipGetLastError());
   CK(exclusive_scan(s, (const int64_t*)W.flag.p, (int64_t*)c.row_p.p, D.n, (int64_t*)W.scan.p));
   int64_t nblk = 0;
-  CK(hipMemcpyAsync(&nblk, (int64_t*)c.row_p.p + D.n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  CK(hipStreamSynchronize(s));
+  CK(read_sizes(ctx, (int64_t*)c.row_p.p + D.n, &nblk, 1));
   c.nblk = nblk;
   CK(c.col.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(nblk, 1)));
   CK(c.off.ensure(sizeof(int64_t) * (size_t)std::max<int64_t>(nblk, 1)));
@@ -759,16 +768,16 @@ hipError_t build_c_index(hipStream_t s, SymScratch& W, const DistView& D, const 
                        (const int64_t*)c.row_p.p, (int32_t*)c.col.p, (int64_t*)c.off.p,
                        (int32_t*)c.rows.p, (unsigned long long*)W.stats.p);
   CK(hipGetLastError());
-  unsigned long long ne = 0;
-  CK(hipMemcpyAsync(&ne, W.stats.p, sizeof(ne), hipMemcpyDeviceToHost, s));
-  CK(hipStreamSynchronize(s));
-  c.nelems = (int64_t)ne;
+  int64_t ne = 0;
+  CK(read_sizes(ctx, W.stats.p, &ne, 1));
+  c.nelems = ne;
   return hipSuccess;
 }
 
 
-hipError_t build_tile_order(hipStream_t s, SymScratch& W, const DistView& D, SymResult& r,
+hipError_t build_tile_order(lamb_hip_ctx* ctx, SymScratch& W, const DistView& D, SymResult& r,
                             int rb) {
+  hipStream_t s = ctx->stream;
   const int32_t nb = D.nbands;
   const int64_t nt = r.n_tiles;
   CK(r.perm.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(nt, 1)));
