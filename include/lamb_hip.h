@@ -195,6 +195,24 @@ int32_t lamb_hip_group_scale_shift(lamb_hip_mat* A, const double* a, const doubl
 int32_t lamb_hip_mcweeny(lamb_hip_ctx* ctx, lamb_hip_mat* P, int32_t steps, double eps_filter,
                          int32_t flags, double* traces, double* idem, lamb_hip_stats* stats);
 
+/* McWeeny chains as HIP graphs. capture records `steps` iterations from P
+ * (sizes the host reads are logged), grows the chain's buffers to their
+ * maximum and captures the same chain on the context stream; P is not
+ * modified. replay runs the graph on P_in (which must have the captured
+ * pattern; its values may differ) and copies the result to P_out, with the
+ * per-step traces / idempotency errors of lamb_hip_mcweeny and the graph's
+ * device time in ms. A replay whose sizes differ from the recorded ones
+ * (other filter decisions) fails with a precondition error. */
+typedef struct lamb_hip_graph lamb_hip_graph;
+int32_t lamb_hip_mcweeny_capture(lamb_hip_ctx* ctx, const lamb_hip_mat* P, int32_t steps,
+                                 double eps_filter, int32_t flags, lamb_hip_graph** graph);
+int32_t lamb_hip_graph_replay(lamb_hip_graph* graph, const lamb_hip_mat* P_in,
+                              lamb_hip_mat* P_out, double* traces, double* idem, double* ms);
+/* steps, number of recorded sizes, flops of one replay */
+int32_t lamb_hip_graph_info(const lamb_hip_graph* graph, int32_t* steps, int64_t* sizes,
+                            double* flops);
+int32_t lamb_hip_graph_destroy(lamb_hip_graph* graph);
+
 /* --- differentiation ------------------------------------------------------
  * No counterpart in the reference. Reverse-mode adjoint of C = A*B: with G
  * the gradient of a loss with respect to C (stored on C's pattern),

@@ -73,6 +73,12 @@ SIGNATURES = {
     "lamb_hip_group_scale_shift": (C.c_int32, [C.c_void_p, f64p, f64p]),
     "lamb_hip_mcweeny": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_int32,
                                      f64p, f64p, C.POINTER(Stats)]),
+    "lamb_hip_mcweeny_capture": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_double,
+                                             C.c_int32, vpp]),
+    "lamb_hip_graph_replay": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, f64p, f64p,
+                                          f64p]),
+    "lamb_hip_graph_info": (C.c_int32, [C.c_void_p, i32p, i64p, f64p]),
+    "lamb_hip_graph_destroy": (C.c_int32, [C.c_void_p]),
     "lamb_hip_multiply_adjoint": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                               C.c_void_p, C.c_void_p, C.c_int32,
                                               C.POINTER(Stats)]),

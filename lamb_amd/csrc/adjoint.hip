@@ -21,12 +21,16 @@ using namespace lamb;
 
 namespace lamb {
 
+LAMB_GUARD_UNIT(adjoint)
+
+
 // one thread per block of XT: XT(j, i) = X(i, j)^T; src_off[b] is the offset
 // of X(i, j) in X's pool
 __global__ __launch_bounds__(256) void transpose_blocks_kernel(
     int64_t nblk, const int32_t* __restrict__ t_rows, const int32_t* __restrict__ t_col,
     const int64_t* __restrict__ t_off, const int64_t* __restrict__ src_off,
     const int32_t* __restrict__ blk, const double* __restrict__ src, double* __restrict__ dst) {
+  LAMB_GUARD;
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nblk) return;
   const int32_t j = t_rows[b], i = t_col[b];
@@ -44,6 +48,7 @@ __global__ __launch_bounds__(256) void project_blocks_kernel(
     const int64_t* __restrict__ l_off, const int64_t* __restrict__ s_row_p,
     const int32_t* __restrict__ s_col, const int64_t* __restrict__ s_off,
     const int32_t* __restrict__ blk, const double* __restrict__ src, double* __restrict__ dst) {
+  LAMB_GUARD;
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nblk) return;
   const int32_t i = l_rows[b], j = l_col[b];
@@ -71,6 +76,7 @@ __global__ __launch_bounds__(256) void project_blocks_kernel(
 __global__ void block_sizes_kernel(int64_t nblk, const int32_t* __restrict__ rows,
                                    const int32_t* __restrict__ col,
                                    const int32_t* __restrict__ blk, int64_t* __restrict__ sz) {
+  LAMB_GUARD;
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b < nblk) sz[b] = (int64_t)blk[rows[b]] * blk[col[b]];
 }
@@ -81,6 +87,7 @@ __global__ __launch_bounds__(256) void move_values_kernel(
     const int64_t* __restrict__ off, const int64_t* __restrict__ voff,
     const int32_t* __restrict__ blk, int to_pool, double* __restrict__ pool,
     double* __restrict__ vals) {
+  LAMB_GUARD;
   const int64_t b = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (b >= nblk) return;

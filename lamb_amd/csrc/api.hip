@@ -91,6 +91,10 @@ int32_t lamb_hip_init(int32_t device, lamb_hip_ctx** out) {
       delete ctx;
       return LAMB_HIP_FATAL;
     }
+  if (install_guard(device, &ctx->guard) != hipSuccess) {
+    delete ctx;
+    return LAMB_HIP_FATAL;
+  }
   *out = ctx;
   return LAMB_HIP_OK;
 }

@@ -33,6 +33,9 @@
 
 namespace lamb {
 
+LAMB_GUARD_UNIT(bf16split)
+
+
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double dbl4 __attribute__((ext_vector_type(4)));
@@ -50,6 +53,7 @@ __device__ __forceinline__ int32_t exp_above(double x) {  // |x| < 2^e
 }
 
 __global__ void fill_exp_kernel(int32_t* __restrict__ ex, int64_t n) {
+  LAMB_GUARD;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) ex[i] = kNoExp;
 }
@@ -61,6 +65,7 @@ __global__ __launch_bounds__(256) void slice_exp_kernel(
     const int32_t* __restrict__ blk, const int64_t* __restrict__ elem0,
     const int64_t* __restrict__ off, const double* __restrict__ data, int row_side,
     int32_t* __restrict__ ex) {
+  LAMB_GUARD;
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nblk) return;
   const int32_t ra = rows[b], ca = cols[b];
@@ -97,6 +102,7 @@ __global__ __launch_bounds__(256) void pack_slices_kernel(
     const Band* __restrict__ bands, const int32_t* __restrict__ blk,
     const int64_t* __restrict__ elem0, const int32_t* __restrict__ ex,
     const double* __restrict__ data, int row_side, int64_t n_chunks, uint4* __restrict__ out) {
+  LAMB_GUARD;
   const int lane = threadIdx.x & 63;
   const int64_t stride = n_chunks + 1;
   if (blockIdx.x == 0 && threadIdx.x < 64) {
@@ -178,6 +184,7 @@ __global__ __launch_bounds__(256, 2) void numeric_bf16_kernel(
     const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
     const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
     double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
+  LAMB_GUARD;
   __shared__ double lds[kBandRows * kPitch];
   uint2* tbl_ab = reinterpret_cast<uint2*>(lds);  // A, B chunk
   uint32_t* tbl_m = reinterpret_cast<uint32_t*>(tbl_ab + kTableChunks);  // rmask | cmask << 16

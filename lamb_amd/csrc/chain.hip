@@ -21,6 +21,9 @@ using namespace lamb;
 
 namespace lamb {
 
+LAMB_GUARD_UNIT(chain)
+
+
 __device__ __forceinline__ double wsum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
@@ -33,6 +36,7 @@ __global__ __launch_bounds__(256) void row_sumsq_kernel(int32_t n, const int64_t
                                                         const int32_t* __restrict__ blk,
                                                         const double* __restrict__ data,
                                                         double* __restrict__ out) {
+  LAMB_GUARD;
   const int32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (i >= n) return;
@@ -54,6 +58,7 @@ __global__ __launch_bounds__(256) void row_diff_sumsq_kernel(
     const int64_t* __restrict__ xo, const double* __restrict__ xd, const int64_t* __restrict__ yr,
     const int32_t* __restrict__ yc, const int64_t* __restrict__ yo, const double* __restrict__ yd,
     const int32_t* __restrict__ blk, double* __restrict__ out) {
+  LAMB_GUARD;
   const int32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (i >= n) return;
@@ -86,6 +91,7 @@ __global__ __launch_bounds__(256) void row_gershgorin_kernel(
     const int64_t* __restrict__ off, const int32_t* __restrict__ blk,
     const double* __restrict__ data, double* __restrict__ lo, double* __restrict__ hi,
     unsigned long long* __restrict__ missing) {
+  LAMB_GUARD;
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int m = blk[i];
@@ -121,6 +127,7 @@ __global__ __launch_bounds__(256) void scale_shift_kernel(
     const int64_t* __restrict__ off, const int32_t* __restrict__ blk,
     const int32_t* __restrict__ group, const double* __restrict__ a,
     const double* __restrict__ b, double* __restrict__ data) {
+  LAMB_GUARD;
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nblk) return;
   const int32_t i = rows[k], j = col[k];
@@ -134,6 +141,34 @@ __global__ __launch_bounds__(256) void scale_shift_kernel(
       if (i == j && r == c) v += t;
       x[(int64_t)c * m + r] = v;
     }
+}
+
+// per-atom partials of ||X - Y||_F^2 (Y == nullptr: ||X||_F^2) into out[n]
+hipError_t launch_diff_partials(hipStream_t s, const lamb_hip_mat* X, const lamb_hip_mat* Y,
+                                double* out) {
+  const lamb_hip_dist* d = X->dist;
+  if (d->n == 0) return hipSuccess;
+  const unsigned grid = (unsigned)((d->n + 3) / 4);
+  if (Y)
+    hipLaunchKernelGGL(row_diff_sumsq_kernel, dim3(grid), dim3(256), 0, s, d->n,
+                       (const int64_t*)X->d_row_p.p, (const int32_t*)X->d_col.p,
+                       (const int64_t*)X->d_off.p, (const double*)X->d_data.p,
+                       (const int64_t*)Y->d_row_p.p, (const int32_t*)Y->d_col.p,
+                       (const int64_t*)Y->d_off.p, (const double*)Y->d_data.p,
+                       (const int32_t*)d->d_blk.p, out);
+  else
+    hipLaunchKernelGGL(row_sumsq_kernel, dim3(grid), dim3(256), 0, s, d->n,
+                       (const int64_t*)X->d_row_p.p, (const int32_t*)X->d_col.p,
+                       (const int64_t*)X->d_off.p, (const int32_t*)d->d_blk.p,
+                       (const double*)X->d_data.p, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_trace_partials(hipStream_t s, const lamb_hip_mat* X, double* out) {
+  const lamb_hip_dist* d = X->dist;
+  return launch_trace(s, d->n, (const int64_t*)X->d_row_p.p, (const int32_t*)X->d_col.p,
+                      (const int64_t*)X->d_off.p, (const int32_t*)d->d_blk.p,
+                      (const double*)X->d_data.p, out);
 }
 
 }  // namespace lamb

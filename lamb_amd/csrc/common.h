@@ -69,6 +69,29 @@ struct Band {
 };
 static_assert(sizeof(Band) == 56, "Band layout");
 
+// Graph replays (graph.hip) must stop all later work once a replayed size
+// differs from the recorded one: every kernel that can run inside a captured
+// chain starts with LAMB_GUARD, which returns when the device's guard flag is
+// raised. Each translation unit holds its own pointer to the flag, set by
+// its lamb_guard_set_<unit> (all of them called by install_guard).
+#define LAMB_GUARD_UNIT(unit)                                                  \
+  static __device__ const unsigned long long* lamb_guard_flag = nullptr;       \
+  hipError_t lamb_guard_set_##unit(const unsigned long long* p) {              \
+    return hipMemcpyToSymbol(HIP_SYMBOL(lamb_guard_flag), &p, sizeof(p));      \
+  }
+#define LAMB_GUARD                                   \
+  do {                                               \
+    if (lamb_guard_flag && *lamb_guard_flag) return; \
+  } while (0)
+hipError_t lamb_guard_set_symbolic(const unsigned long long* p);
+hipError_t lamb_guard_set_scan(const unsigned long long* p);
+hipError_t lamb_guard_set_kernels(const unsigned long long* p);
+hipError_t lamb_guard_set_bf16split(const unsigned long long* p);
+hipError_t lamb_guard_set_chain(const unsigned long long* p);
+hipError_t lamb_guard_set_adjoint(const unsigned long long* p);
+// the guard flag of a device (allocated and installed on first use)
+hipError_t install_guard(int device, unsigned long long** flag);
+
 // Kernel launchers (kernels.hip).
 // Pack strips into dense [kdim][64] panels: A strips put band rows along the
 // 64 (row_side = true), B strips put band columns along the 64.

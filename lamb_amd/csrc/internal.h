@@ -46,6 +46,12 @@ int32_t multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A, const lamb_hip
                         lamb_hip_mat* C, double alpha, double beta, double eps, int32_t flags,
                         lamb_hip_stats* stats, const lamb_hip_mat* mask);
 
+// Per-atom partials (chain.hip): ||X - Y||_F^2 (Y may be nullptr) and the
+// diagonal sums, written to out[n_atoms] in internal atom order.
+hipError_t launch_diff_partials(hipStream_t s, const lamb_hip_mat* X, const lamb_hip_mat* Y,
+                                double* out);
+hipError_t launch_trace_partials(hipStream_t s, const lamb_hip_mat* X, double* out);
+
 // A matrix whose value pool was released (lamb_hip_matrix_release_values)
 // while it stores blocks: entry points that read values refuse it.
 bool values_missing(const lamb_hip_mat* m);
@@ -81,7 +87,7 @@ struct lamb_hip_ctx {
   int32_t size_mode = 0;
   std::vector<int64_t> size_log;
   size_t size_cursor = 0;
-  lamb::DevBuf d_guard;  // int64: a replayed size differed from the recorded one
+  unsigned long long* guard = nullptr;  // the device's guard flag (install_guard)
 };
 
 namespace lamb {

@@ -4,6 +4,9 @@
 
 namespace lamb {
 
+LAMB_GUARD_UNIT(kernels)
+
+
 // ---------------------------------------------------------------------------
 // Block norms: nsq(X) = sum of squares in column-major order, one FMA per
 // step, starting from +0. One thread owns one block so the summation order
@@ -15,6 +18,7 @@ __global__ __launch_bounds__(256) void block_nsq_kernel(int64_t nblk, const int3
                                                         const int64_t* __restrict__ off,
                                                         const double* __restrict__ data,
                                                         double* __restrict__ nsq) {
+  LAMB_GUARD;
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nblk) return;
   const int64_t len = (int64_t)blk[rows[b]] * blk[cols[b]];
@@ -49,6 +53,7 @@ __global__ __launch_bounds__(256) void pack_strips_kernel(int64_t n_strips,
                                                           const int32_t* __restrict__ blk,
                                                           const double* __restrict__ data,
                                                           int row_side, double* __restrict__ packed) {
+  LAMB_GUARD;
   const int lane = threadIdx.x & 63;
   // the panel column after the last strip is all zeros (the numeric kernel
   // points masked-off operand lanes at it)
@@ -171,6 +176,7 @@ __global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
     const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
     const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
     double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
+  LAMB_GUARD;
   // main loop: the step table of a window of K columns; epilogue: C tile
   __shared__ double lds[kBandRows * kPitch];
   uint2* col_ab = reinterpret_cast<uint2*>(lds);  // A, B panel column
@@ -325,6 +331,7 @@ __global__ __launch_bounds__(256, 2) void numeric_f64_ksplit_kernel(
     const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
     const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
     double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
+  LAMB_GUARD;
   __shared__ double lds[kBandRows * kPitch];
   uint2* col_ab = reinterpret_cast<uint2*>(lds);
   uint32_t* col_m = reinterpret_cast<uint32_t*>(col_ab + kTableCols);
@@ -477,6 +484,7 @@ __global__ void trace_kernel(int32_t n_atoms, const int64_t* __restrict__ row_p,
                              const int32_t* __restrict__ col, const int64_t* __restrict__ off,
                              const int32_t* __restrict__ blk, const double* __restrict__ data,
                              double* __restrict__ out) {
+  LAMB_GUARD;
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_atoms) return;
   double t = 0.0;

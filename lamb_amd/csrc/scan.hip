@@ -5,12 +5,17 @@
 
 namespace lamb {
 
+LAMB_GUARD_UNIT(scan)
+
+
 __global__ void fill_i64(int64_t* __restrict__ p, int64_t n, int64_t v) {
+  LAMB_GUARD;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
 
 __global__ void copy_i64(int64_t* __restrict__ dst, const int64_t* __restrict__ src, int64_t n) {
+  LAMB_GUARD;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = src[i];
 }
@@ -44,6 +49,7 @@ __device__ int64_t block_exclusive_scan(int64_t v, int64_t* total) {
 __global__ __launch_bounds__(kScanBlock) void scan_tiles(const int64_t* __restrict__ in,
                                                          int64_t* __restrict__ out, int64_t n,
                                                          int64_t* __restrict__ sums) {
+  LAMB_GUARD;
   const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
   int64_t v[kScanItems];
   int64_t t = 0;
@@ -64,6 +70,7 @@ __global__ __launch_bounds__(kScanBlock) void scan_tiles(const int64_t* __restri
 
 __global__ __launch_bounds__(kScanBlock) void add_tile_base(int64_t* __restrict__ out, int64_t n,
                                                             const int64_t* __restrict__ bases) {
+  LAMB_GUARD;
   const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
   const int64_t add = bases[blockIdx.x];
 #pragma unroll

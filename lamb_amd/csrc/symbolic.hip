@@ -8,7 +8,11 @@
 
 namespace lamb {
 
+LAMB_GUARD_UNIT(symbolic)
+
+
 __global__ void fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
+  LAMB_GUARD;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
@@ -39,6 +43,7 @@ __global__ void a_merge_rank(int64_t nblk, const int32_t* __restrict__ rows,
                              const int32_t* __restrict__ band_of,
                              const int32_t* __restrict__ pos_in_band,
                              const Band* __restrict__ bands, int64_t* __restrict__ merged) {
+  LAMB_GUARD;
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nblk) return;
   const int32_t i = rows[b];
@@ -66,6 +71,7 @@ __global__ void a_merge_rank(int64_t nblk, const int32_t* __restrict__ rows,
 __global__ void strip_flags(int64_t nblk, int a_side, const int64_t* __restrict__ merged,
                             const int32_t* __restrict__ rows, const int32_t* __restrict__ col,
                             const int32_t* __restrict__ band_of, int64_t* __restrict__ flag) {
+  LAMB_GUARD;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nblk) return;
   int64_t f;
@@ -84,6 +90,7 @@ __global__ void strip_flags(int64_t nblk, int a_side, const int64_t* __restrict_
 }
 
 __global__ void strip_init(int64_t n, Strip* __restrict__ s) {
+  LAMB_GUARD;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   s[i].mask = 0;
@@ -99,6 +106,7 @@ __global__ void strip_fill(int64_t nblk, int a_side, const int64_t* __restrict__
                            const int64_t* __restrict__ off, const double* __restrict__ nsq,
                            const int32_t* __restrict__ band_of,
                            const int32_t* __restrict__ pos_in_band, Strip* __restrict__ strips) {
+  LAMB_GUARD;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nblk) return;
   const int64_t b = a_side ? merged[t] : t;
@@ -122,6 +130,7 @@ __global__ void strip_fill(int64_t nblk, int a_side, const int64_t* __restrict__
 // slice chunk), times 64
 __global__ void strip_panel_size(int64_t n, const Strip* __restrict__ s,
                                  const int32_t* __restrict__ blk, int64_t* __restrict__ sz) {
+  LAMB_GUARD;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) sz[i] = 64 * (int64_t)((blk[s[i].k] + 7) & ~7);
 }
@@ -130,6 +139,7 @@ __global__ void strip_panel_size(int64_t n, const Strip* __restrict__ s,
 __global__ void group_ptr(int64_t ngroups, const int64_t* __restrict__ slot_of_group,
                           const int64_t* __restrict__ excl, int64_t nslots, int64_t nstrips,
                           int64_t* __restrict__ ptr) {
+  LAMB_GUARD;
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g > ngroups) return;
   if (g == ngroups) {
@@ -143,6 +153,7 @@ __global__ void group_ptr(int64_t ngroups, const int64_t* __restrict__ slot_of_g
 // slot of band I's first block in the merged A order = row_p[first atom]
 __global__ void band_first_slot(int32_t nbands, const Band* __restrict__ bands,
                                 const int64_t* __restrict__ row_p, int64_t* __restrict__ out) {
+  LAMB_GUARD;
   const int32_t I = blockIdx.x * blockDim.x + threadIdx.x;
   if (I < nbands) out[I] = row_p[bands[I].first];
 }
@@ -231,6 +242,7 @@ __global__ void mask_pattern(int64_t nblk, const int32_t* __restrict__ rows,
                              const int32_t* __restrict__ col, const int32_t* __restrict__ band_of,
                              const int32_t* __restrict__ pos_in_band, int32_t nbands,
                              uint32_t* __restrict__ mpat) {
+  LAMB_GUARD;
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nblk) return;
   const int32_t i = rows[b], j = col[b];
@@ -247,6 +259,7 @@ __global__ __launch_bounds__(64) void sym_count(
     const int32_t* __restrict__ band_of, const int32_t* __restrict__ pos_in_band,
     int32_t* __restrict__ cnt, int32_t* __restrict__ ksum, uint32_t* __restrict__ pat,
     unsigned long long* __restrict__ stats, const uint32_t* __restrict__ mpat) {
+  LAMB_GUARD;
   const int64_t w = blockIdx.x;
   const int32_t I = (int32_t)(w / nseg);
   const int g = (int)(w % nseg);
@@ -323,6 +336,7 @@ __global__ __launch_bounds__(64) void sym_band_totals(
     const uint32_t* __restrict__ pat, int64_t* __restrict__ band_tiles,
     int64_t* __restrict__ band_entries, int64_t* __restrict__ band_csize,
     int64_t* __restrict__ band_cblocks) {
+  LAMB_GUARD;
   const int32_t I = blockIdx.x;
   if (I >= nbands) return;
   const int lane = threadIdx.x;
@@ -369,6 +383,7 @@ __global__ __launch_bounds__(64) void sym_tiles(
     int32_t* __restrict__ ksum, const uint32_t* __restrict__ pat,
     const int64_t* __restrict__ tile_base, const int64_t* __restrict__ entry_base,
     const int64_t* __restrict__ csize_base, TileDesc* __restrict__ tiles) {
+  LAMB_GUARD;
   const int32_t I = blockIdx.x;
   if (I >= nbands) return;
   const int lane = threadIdx.x;
@@ -435,6 +450,7 @@ __global__ __launch_bounds__(64) void sym_fill(
     const int64_t* __restrict__ b_ptr, const int64_t* __restrict__ a_pk,
     const int64_t* __restrict__ b_pk, double eps2, int32_t* __restrict__ cursor,
     int32_t* __restrict__ kcur, Entry* __restrict__ entries, const uint32_t* __restrict__ mpat) {
+  LAMB_GUARD;
   const int64_t w = blockIdx.x;
   const int32_t I = (int32_t)(w / nseg);
   const int g = (int)(w % nseg);
@@ -489,6 +505,7 @@ __global__ __launch_bounds__(64) void cidx_count(int32_t nbands, const Band* __r
                                                  const int64_t* __restrict__ tile_base,
                                                  const uint16_t* __restrict__ kept,
                                                  int64_t* __restrict__ row_cnt) {
+  LAMB_GUARD;
   const int32_t I = blockIdx.x;
   if (I >= nbands) return;
   const int lane = threadIdx.x;
@@ -508,6 +525,7 @@ __global__ __launch_bounds__(64) void cidx_fill(int32_t nbands, const Band* __re
                                                 int32_t* __restrict__ col, int64_t* __restrict__ off,
                                                 int32_t* __restrict__ rows,
                                                 unsigned long long* __restrict__ nelems) {
+  LAMB_GUARD;
   const int32_t I = blockIdx.x;
   if (I >= nbands) return;
   const int lane = threadIdx.x;
@@ -557,6 +575,7 @@ __global__ __launch_bounds__(64) void cidx_fill(int32_t nbands, const Band* __re
 // ---------------------------------------------------------------------------
 __global__ void tile_cells(int64_t ntiles, int rb, int32_t nbands, const TileDesc* __restrict__ tiles,
                            uint32_t* __restrict__ cell_mask) {
+  LAMB_GUARD;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntiles) return;
   const int32_t I = tiles[t].I, J = tiles[t].J;
@@ -565,6 +584,7 @@ __global__ void tile_cells(int64_t ntiles, int rb, int32_t nbands, const TileDes
 
 __global__ void cell_counts(int64_t ncells, const uint32_t* __restrict__ cell_mask,
                             int64_t* __restrict__ cnt) {
+  LAMB_GUARD;
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c < ncells) cnt[c] = __builtin_popcount(cell_mask[c]);
 }
@@ -572,6 +592,7 @@ __global__ void cell_counts(int64_t ncells, const uint32_t* __restrict__ cell_ma
 __global__ void tile_perm(int64_t ntiles, int rb, int32_t nbands, const TileDesc* __restrict__ tiles,
                           const uint32_t* __restrict__ cell_mask, const int64_t* __restrict__ base,
                           int32_t* __restrict__ perm) {
+  LAMB_GUARD;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntiles) return;
   const int32_t I = tiles[t].I, J = tiles[t].J;
@@ -584,6 +605,7 @@ __global__ void tile_perm(int64_t ntiles, int rb, int32_t nbands, const TileDesc
 // out[4..6] = kept, dropped, flops statistics
 __global__ void gather_totals(const int64_t* __restrict__ bts, int32_t nb,
                               const int64_t* __restrict__ stats, int64_t* __restrict__ out) {
+  LAMB_GUARD;
   const int t = threadIdx.x;
   if (t < 4) out[t] = bts[(int64_t)t * (nb + 2) + nb];
   else if (t < 7) out[t] = stats[t - 4];

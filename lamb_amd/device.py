@@ -221,6 +221,44 @@ def mcweeny(p: DeviceMatrix, steps: int, *, eps_filter: float = 0.0, flags: int 
     return traces, idem, st.as_dict()
 
 
+class ChainGraph:
+    """A McWeeny chain captured as a HIP graph (lamb_hip_mcweeny_capture)."""
+
+    def __init__(self, p: DeviceMatrix, steps: int, *, eps_filter: float = 0.0,
+                 flags: int = FP64):
+        self.ctx = p.ctx
+        self.steps = int(steps)
+        self.n_groups = p.ddist.n_groups
+        h = C.c_void_p()
+        rc = self.ctx.lib.lamb_hip_mcweeny_capture(self.ctx.h, p.h, self.steps, float(eps_filter),
+                                                   int(flags), C.byref(h))
+        self.ctx.check(rc, "mcweeny_capture")
+        self.h = h
+
+    def replay(self, p_in: DeviceMatrix, p_out: DeviceMatrix):
+        """Run the graph on p_in (the captured pattern) into p_out; returns
+        (traces, idempotency errors) [steps, n_groups] and the device ms."""
+        traces = np.zeros((self.steps, self.n_groups))
+        idem = np.zeros((self.steps, self.n_groups))
+        ms = C.c_double()
+        rc = self.ctx.lib.lamb_hip_graph_replay(self.h, p_in.h, p_out.h,
+                                                N.ptr(traces, C.c_double),
+                                                N.ptr(idem, C.c_double), C.byref(ms))
+        self.ctx.check(rc, "graph_replay")
+        return traces, idem, ms.value
+
+    def info(self):
+        steps, sizes, flops = C.c_int32(), C.c_int64(), C.c_double()
+        self.ctx.check(self.ctx.lib.lamb_hip_graph_info(self.h, C.byref(steps), C.byref(sizes),
+                                                        C.byref(flops)), "graph_info")
+        return dict(steps=steps.value, recorded_sizes=sizes.value, flops=flops.value)
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.lamb_hip_graph_destroy(self.h)
+            self.h = None
+
+
 def multiply(a: DeviceMatrix, b: DeviceMatrix, c: DeviceMatrix, *, alpha: float = 1.0,
              beta: float = 0.0, eps_filter: float = 0.0, flags: int = FP64) -> dict:
     """C = alpha*A*B + beta*C on the device; returns the call's statistics."""
