@@ -36,6 +36,18 @@ def parse():
                    help="CPU baseline samples every n-th block row of A")
     p.add_argument("--slices", type=int, default=5, help="BF16 slices of the split form")
     p.add_argument("--no-bf16", action="store_true", help="skip the BF16-split leg")
+    p.add_argument("--workload", default="water6144",
+                   choices=["water6144", "config3", "config4", "config5"],
+                   help="water6144: the headline multiply; config3: BF16 split on the "
+                        "512-atom box; config4: batched McWeeny chains (eager and graph); "
+                        "config5: gradient of tr(P_k H) through the adjoint kernels")
+    p.add_argument("--systems", type=int, default=64, help="config4: systems in the batch")
+    p.add_argument("--lo", type=int, default=96, help="config4: smallest system (atoms)")
+    p.add_argument("--hi", type=int, default=3072, help="config4: largest system (atoms)")
+    p.add_argument("--chain-steps", type=int, default=0,
+                   help="McWeeny steps (default 20 for config4, 10 for config5)")
+    p.add_argument("--atoms", type=int, default=3072, help="config5: atoms of the water box")
+    p.add_argument("--fd-blocks", type=int, default=0, help="config5: blocks checked by FD")
     return p.parse_args()
 
 
@@ -96,12 +108,185 @@ def bf16_leg(D, da, db, dc, dd, eps, args, ms_step64, ms_num64):
     }
 
 
+def line(metric, value, unit, ms, args, config, dtype="f64", **extra):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": dtype, "data": "synthetic (seeded water-like atom box)",
+           "config": config}
+    out.update(extra)
+    return out
+
+
+def run_config3(args):
+    """BF16 split (and FP64) on the 512-atom box, errors against the CPU oracle."""
+    import oracle
+    from __graft_entry__ import compare
+    from lamb_amd import device as D
+    from lamb_amd import systems as S
+    from lamb_amd.matrix import Distribution, LambMatrix
+    atoms, a, b, eps = S.config1(args.seed if args.seed != 2 else 1)
+    ref, _ = oracle.multiply(a, b, eps=eps)
+    dist = Distribution(atoms.blk, atoms.pos, atoms.box)
+    ctx = D.Context(0)
+    dd = D.DeviceDistribution(ctx, dist)
+    da = D.DeviceMatrix.from_lamb(dd, LambMatrix.from_global(dist, a))
+    db = D.DeviceMatrix.from_lamb(dd, LambMatrix.from_global(dist, b))
+    res = {}
+    for name, flags in (("fp64", D.FP64), ("bf16_split", D.bf16_split(args.slices))):
+        dc = D.DeviceMatrix(dd)
+        for _ in range(max(args.warmup, 1)):
+            D.multiply(da, db, dc, eps_filter=eps, flags=flags)
+        t0 = time.perf_counter()
+        sts = [D.multiply(da, db, dc, eps_filter=eps, flags=flags) for _ in range(args.steps)]
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        same, err = compare(dc.download().to_global(), ref)
+        res[name] = {"gflops": sts[-1]["flops"] / (ms * 1e-3) / 1e9, "ms_per_step": ms,
+                     "ms_numeric": float(np.mean([x["ms_numeric"] for x in sts])),
+                     "rel_frobenius_error_vs_oracle": err, "pattern_identical": bool(same)}
+        dc.close()
+    for h in (db, da, dd):
+        h.close()
+    ctx.close()
+    r = res["bf16_split"]
+    return line("block-SpGEMM GFLOP/s, BF16 split vs FP64 on the 512-atom box", r["gflops"],
+                "GFLOP/s", r["ms_per_step"], args,
+                {"workload": "512 atoms, U[0,17.4)^3, block 13, r<6 A, exp(-0.5r), A=B "
+                             "symmetric + 2I, eps 1e-8", "slices": args.slices, **res},
+                dtype="bf16x%d-split/f32-mfma/f64-accumulate" % args.slices)
+
+
+def run_config4(args):
+    """A batch of independent systems, McWeeny chains eager and as a HIP graph."""
+    from lamb_amd import device as D
+    from lamb_amd import systems as S
+    from lamb_amd.matrix import Distribution, LambMatrix
+    steps = args.chain_steps or 20
+    eps = 1e-7
+    t0 = time.perf_counter()
+    bt = S.config4(seed=4, n_systems=args.systems, lo=args.lo, hi=args.hi)
+    sizes = [a.n_atoms for a, _ in bt.systems]
+    print(f"batch of {bt.n_systems} systems, {bt.atoms.n_atoms} atoms ({min(sizes)}..{max(sizes)}),"
+          f" built in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    dist = Distribution(bt.atoms.blk, bt.atoms.pos, bt.atoms.box)
+    ctx = D.Context(0)
+    dd = D.DeviceDistribution(ctx, dist)
+    dd.set_groups(bt.group, bt.n_systems)
+    dh = D.DeviceMatrix.from_lamb(dd, LambMatrix.from_global(dist, bt.h))
+    flags = D.FP64
+    p = D.purification_start(dh)
+    t0 = time.perf_counter()
+    tr, idem, st = D.mcweeny(p, steps, eps_filter=eps, flags=flags)
+    ms_eager = (time.perf_counter() - t0) * 1e3
+    print(f"eager chain: {ms_eager:.0f} ms, {st['flops']:.3e} flop", file=sys.stderr, flush=True)
+    p0 = D.purification_start(dh)
+    t0 = time.perf_counter()
+    g = D.ChainGraph(p0, steps, eps_filter=eps, flags=flags)
+    ms_capture = (time.perf_counter() - t0) * 1e3
+    out = D.DeviceMatrix(dd)
+    reps = []
+    for _ in range(max(args.steps // 5, 2)):
+        tg, ig, ms = g.replay(p0, out)
+        reps.append(ms)
+    same = bool(np.array_equal(tg, tr) and np.array_equal(ig, idem))
+    flops = g.info()["flops"]
+    ms_graph = float(np.median(reps))
+    occ = np.round(tr[-1]).astype(int)
+    for h in (g, out, p0, p, dh, dd):
+        h.close()
+    v, ncls = D.tuning_info(ctx)
+    ctx.close()
+    return line("batched McWeeny chains: GFLOP/s of the chain replayed as a HIP graph",
+                flops / (ms_graph * 1e-3) / 1e9, "GFLOP/s", ms_graph, args,
+                {"workload": f"{bt.n_systems} water-like systems, {min(sizes)}..{max(sizes)} atoms"
+                             f" (log-uniform, seed 4), O 13 / H 5 / 23 for 10%, r<8 A, "
+                             f"{steps} McWeeny steps from H scaled to [0,1], eps 1e-7",
+                 "atoms": int(bt.atoms.n_atoms), "chain_steps": steps,
+                 "flops_per_chain": flops, "ms_eager_chain": ms_eager,
+                 "gflops_eager": flops / (ms_eager * 1e-3) / 1e9, "ms_graph_replay": ms_graph,
+                 "ms_capture": ms_capture, "graph_equals_eager_bitwise": same,
+                 "max_final_idempotency_error": float(idem[-1].max()),
+                 "occupied_counts_first_systems": occ[:8].tolist(),
+                 "autotuned_classes": ncls})
+
+
+def run_config5(args):
+    """Gradient of L = tr(P_k H) through the adjoint kernels (torch.autograd)."""
+    import torch
+    from lamb_amd import autograd as AG
+    from lamb_amd import device as D
+    from lamb_amd import systems as S
+    from lamb_amd.matrix import Distribution, LambMatrix
+    steps = args.chain_steps or 10
+    atoms = S.water_system(args.atoms, seed=5)
+    h = S.decay_matrix(atoms, 8.0, 0.4, 6, symmetric=True)
+    dist = Distribution(atoms.blk, atoms.pos, atoms.box)
+    ctx = D.Context(0)
+    dd = D.DeviceDistribution(ctx, dist)
+    dh = D.DeviceMatrix.from_lamb(dd, LambMatrix.from_global(dist, h))
+    lo, hi = (float(x[0]) for x in dh.group_gershgorin())
+    hv, ph = AG.leaf(dh)
+    res = {}
+    grads = {}
+    for name, flags in (("fp64", D.FP64), ("bf16_split", D.bf16_split(args.slices))):
+        # one untimed pass (autotuning, first torch calls)
+        w = hv.clone().requires_grad_(True)
+        AG.purification_loss(w, ph, atoms.blk, lo, hi, min(steps, 2), flags=flags).backward()
+        x = hv.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        loss = AG.purification_loss(x, ph, atoms.blk, lo, hi, steps, flags=flags)
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        grads[name] = x.grad.detach()
+        res[name] = {"loss": float(loss), "ms_forward": (t1 - t0) * 1e3,
+                     "ms_forward_backward": (t2 - t0) * 1e3}
+        print(f"{name}: loss {float(loss):.10g}, forward {res[name]['ms_forward']:.0f} ms, "
+              f"forward+backward {res[name]['ms_forward_backward']:.0f} ms", file=sys.stderr,
+              flush=True)
+    gdiff = float(torch.linalg.norm(grads["bf16_split"] - grads["fp64"]) /
+                  torch.linalg.norm(grads["fp64"]))
+    fd = None
+    if args.fd_blocks:
+        rows, cols = ph.block_atoms()
+        sz = atoms.blk[rows].astype(np.int64) * atoms.blk[cols]
+        offs = np.concatenate([[0], np.cumsum(sz)])
+        rng = np.random.default_rng(7)
+        worst = 0.0
+        with torch.no_grad():
+            for k in rng.choice(len(rows), size=args.fd_blocks, replace=False):
+                d = torch.zeros_like(hv)
+                v = rng.normal(size=int(sz[k]))
+                d[offs[k]:offs[k + 1]] = torch.from_numpy(v / np.linalg.norm(v)).to(d.device)
+                lp = AG.purification_loss(hv + 1e-6 * d, ph, atoms.blk, lo, hi, steps).item()
+                lm = AG.purification_loss(hv - 1e-6 * d, ph, atoms.blk, lo, hi, steps).item()
+                an = float(torch.dot(grads["fp64"], d))
+                worst = max(worst, abs((lp - lm) / 2e-6 - an) / max(abs(an), 1.0))
+        fd = worst
+    dd.close()
+    ctx.close()
+    r = res["fp64"]
+    return line("gradient of tr(P_k H) through hand-written adjoint kernels: ms per "
+                "forward+backward", r["ms_forward_backward"], "ms", r["ms_forward_backward"],
+                args, {"workload": f"{atoms.n_atoms}-atom water box, O 13 / H 5, r<8 A, "
+                                   f"L = tr(P_{steps} H), {steps} McWeeny steps, checkpoint "
+                                   f"every 2", **res,
+                       "grad_rel_diff_bf16_vs_fp64": gdiff,
+                       "fd_blocks": args.fd_blocks, "fd_worst_scaled_error": fd},
+                higher_is_better=False)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the sector multiply is measured on one GPU's share)", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "water6144":
+        fn = {"config3": run_config3, "config4": run_config4, "config5": run_config5}
+        print(json.dumps(fn[args.workload](args)), flush=True)
+        return
     from lamb_amd import device as D
     from lamb_amd import systems as S
     from lamb_amd.matrix import Distribution, LambMatrix

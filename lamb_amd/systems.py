@@ -187,16 +187,30 @@ def decay_matrix(atoms: AtomBox, cutoff: float, decay: float, seed: int,
         order = np.argsort(key)
         tpos = order[np.searchsorted(key[order], tkey)]
         uidx = np.flatnonzero(upper)
-        for t, b in enumerate(uidx):
-            mm, kk = int(m[b]), int(k[b])
-            x = vals[u_off[t]:u_off[t + 1]].reshape(kk, mm).T  # m x k
-            if i[b] == j[b]:
-                x = 0.5 * (x + x.T) + diag_shift * np.eye(mm)
-                data[blk_off[b]:blk_off[b] + mm * kk] = x.T.reshape(-1)
-            else:
-                data[blk_off[b]:blk_off[b] + mm * kk] = x.T.reshape(-1)
-                tb = tpos[b]
-                data[blk_off[tb]:blk_off[tb] + mm * kk] = x.reshape(-1)
+        # upper block t is x = seg.reshape(k, m).T (m x k, seg its values):
+        # stored column-major it is seg itself; its mirror (j, i) = x^T;
+        # diagonal blocks become (x + x^T) / 2 + diag_shift * I. Blocks of one
+        # (m, k) shape are handled together.
+        bm, bk = m[uidx], k[uidx]
+        diag = i[uidx] == j[uidx]
+        for mm, kk in set(zip(bm.tolist(), bk.tolist())):
+            sel = np.flatnonzero((bm == mm) & (bk == kk))
+            if sel.size == 0 or mm * kk == 0:
+                continue
+            segs = vals[u_off[sel][:, None] + np.arange(mm * kk)].reshape(-1, kk, mm)
+            x = segs.transpose(0, 2, 1)  # [blocks, m, k]
+            d = diag[sel]
+            if d.any():
+                xd = x[d]
+                xd = 0.5 * (xd + xd.transpose(0, 2, 1)) + diag_shift * np.eye(mm)
+                dst = blk_off[uidx[sel[d]]][:, None] + np.arange(mm * kk)
+                data[dst] = xd.transpose(0, 2, 1).reshape(-1, mm * kk)
+            o = ~d
+            if o.any():
+                dst = blk_off[uidx[sel[o]]][:, None] + np.arange(mm * kk)
+                data[dst] = segs[o].reshape(-1, mm * kk)
+                tdst = blk_off[tpos[uidx[sel[o]]]][:, None] + np.arange(mm * kk)
+                data[tdst] = x[o].reshape(-1, mm * kk)
     return BlockCSR(blk.astype(np.int32), row_p, j.astype(np.int32),
                     blk_off.astype(np.int64), data, r)
 
