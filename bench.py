@@ -48,6 +48,8 @@ def parse():
                    help="McWeeny steps (default 20 for config4, 10 for config5)")
     p.add_argument("--atoms", type=int, default=3072, help="config5: atoms of the water box")
     p.add_argument("--fd-blocks", type=int, default=0, help="config5: blocks checked by FD")
+    p.add_argument("--fd-first", type=int, default=0,
+                   help="config5: first of the 32 drawn blocks checked by this run")
     return p.parse_args()
 
 
@@ -155,6 +157,17 @@ def run_config3(args):
                 dtype="bf16x%d-split/f32-mfma/f64-accumulate" % args.slices)
 
 
+def batch_distribution(atoms, Distribution):
+    """The spatial partition with the fewest subsets whose local indices fit
+    int16 (a batch of many systems can exceed 32,767 atoms in one subset)."""
+    for n_procs in (1, 2, 4, 8, 16, 32, 64):
+        try:
+            return Distribution(atoms.blk, atoms.pos, atoms.box, n_procs=n_procs)
+        except ValueError:
+            continue
+    raise ValueError("no spatial partition keeps every subset within the int16 local index")
+
+
 def run_config4(args):
     """A batch of independent systems, McWeeny chains eager and as a HIP graph."""
     from lamb_amd import device as D
@@ -167,7 +180,7 @@ def run_config4(args):
     sizes = [a.n_atoms for a, _ in bt.systems]
     print(f"batch of {bt.n_systems} systems, {bt.atoms.n_atoms} atoms ({min(sizes)}..{max(sizes)}),"
           f" built in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
-    dist = Distribution(bt.atoms.blk, bt.atoms.pos, bt.atoms.box)
+    dist = batch_distribution(bt.atoms, Distribution)
     ctx = D.Context(0)
     dd = D.DeviceDistribution(ctx, dist)
     dd.set_groups(bt.group, bt.n_systems)
@@ -182,16 +195,47 @@ def run_config4(args):
     t0 = time.perf_counter()
     g = D.ChainGraph(p0, steps, eps_filter=eps, flags=flags)
     ms_capture = (time.perf_counter() - t0) * 1e3
+    print(f"graph capture: {ms_capture:.0f} ms", file=sys.stderr, flush=True)
     out = D.DeviceMatrix(dd)
     reps = []
-    for _ in range(max(args.steps // 5, 2)):
+    for _ in range(max(args.steps // 5, 1)):
         tg, ig, ms = g.replay(p0, out)
         reps.append(ms)
+        print(f"graph replay: {ms:.0f} ms", file=sys.stderr, flush=True)
     same = bool(np.array_equal(tg, tr) and np.array_equal(ig, idem))
     flops = g.info()["flops"]
     ms_graph = float(np.median(reps))
     occ = np.round(tr[-1]).astype(int)
-    for h in (g, out, p0, p, dh, dd):
+    g.close()
+    out.close()
+    bf = None
+    if not args.no_bf16:
+        # the same chains in the BF16 split form, eager and as a graph; the
+        # final iterates are compared with the FP64 ones per system
+        fb = D.bf16_split(args.slices)
+        pb = D.purification_start(dh)
+        t0 = time.perf_counter()
+        trb, idb, stb = D.mcweeny(pb, steps, eps_filter=eps, flags=fb)
+        ms_eager_b = (time.perf_counter() - t0) * 1e3
+        diff = pb.group_diff_norm(p)
+        ref = p.group_norm()
+        gb = D.ChainGraph(p0, steps, eps_filter=eps, flags=fb)
+        outb = D.DeviceMatrix(dd)
+        repb = [gb.replay(p0, outb)[2] for _ in range(max(args.steps // 5, 1))]
+        ms_graph_b = float(np.median(repb))
+        flops_b = gb.info()["flops"]
+        bf = {"slices": args.slices, "ms_eager_chain": ms_eager_b, "ms_graph_replay": ms_graph_b,
+              "gflops_graph": flops_b / (ms_graph_b * 1e-3) / 1e9,
+              "speedup_graph_vs_fp64": ms_graph / ms_graph_b,
+              "max_rel_frobenius_diff_final_p_vs_fp64": float(np.max(diff / np.maximum(ref, 1e-300))),
+              "same_block_count": pb.info()[0] == p.info()[0],
+              "max_trace_diff_final": float(np.max(np.abs(trb[-1] - tr[-1])))}
+        print(f"bf16 x{args.slices}: eager {ms_eager_b:.0f} ms, graph {ms_graph_b:.1f} ms, "
+              f"final-P diff {bf['max_rel_frobenius_diff_final_p_vs_fp64']:.2e}", file=sys.stderr,
+              flush=True)
+        for h in (gb, outb, pb):
+            h.close()
+    for h in (p0, p, dh, dd):
         h.close()
     v, ncls = D.tuning_info(ctx)
     ctx.close()
@@ -206,7 +250,7 @@ def run_config4(args):
                  "ms_capture": ms_capture, "graph_equals_eager_bitwise": same,
                  "max_final_idempotency_error": float(idem[-1].max()),
                  "occupied_counts_first_systems": occ[:8].tolist(),
-                 "autotuned_classes": ncls})
+                 "autotuned_classes": ncls, "bf16_split": bf})
 
 
 def run_config5(args):
@@ -227,7 +271,10 @@ def run_config5(args):
     hv, ph = AG.leaf(dh)
     res = {}
     grads = {}
-    for name, flags in (("fp64", D.FP64), ("bf16_split", D.bf16_split(args.slices))):
+    legs = [("fp64", D.FP64)]
+    if not args.no_bf16:
+        legs.append(("bf16_split", D.bf16_split(args.slices)))
+    for name, flags in legs:
         # one untimed pass (autotuning, first torch calls)
         w = hv.clone().requires_grad_(True)
         AG.purification_loss(w, ph, atoms.blk, lo, hi, min(steps, 2), flags=flags).backward()
@@ -240,29 +287,36 @@ def run_config5(args):
         torch.cuda.synchronize()
         t2 = time.perf_counter()
         grads[name] = x.grad.detach()
-        res[name] = {"loss": float(loss), "ms_forward": (t1 - t0) * 1e3,
+        res[name] = {"loss": float(loss.detach()), "ms_forward": (t1 - t0) * 1e3,
                      "ms_forward_backward": (t2 - t0) * 1e3}
-        print(f"{name}: loss {float(loss):.10g}, forward {res[name]['ms_forward']:.0f} ms, "
+        print(f"{name}: loss {float(loss.detach()):.10g}, forward {res[name]['ms_forward']:.0f} ms, "
               f"forward+backward {res[name]['ms_forward_backward']:.0f} ms", file=sys.stderr,
               flush=True)
-    gdiff = float(torch.linalg.norm(grads["bf16_split"] - grads["fp64"]) /
-                  torch.linalg.norm(grads["fp64"]))
+    gdiff = None
+    if "bf16_split" in grads:
+        gdiff = float(torch.linalg.norm(grads["bf16_split"] - grads["fp64"]) /
+                      torch.linalg.norm(grads["fp64"]))
     fd = None
     if args.fd_blocks:
         rows, cols = ph.block_atoms()
         sz = atoms.blk[rows].astype(np.int64) * atoms.blk[cols]
         offs = np.concatenate([[0], np.cumsum(sz)])
-        rng = np.random.default_rng(7)
+        # 32 blocks drawn once; this run checks picks [fd_first, fd_first + fd_blocks)
+        picks = np.random.default_rng(7).choice(len(rows), size=max(32, args.fd_blocks),
+                                                replace=False)
         worst = 0.0
         with torch.no_grad():
-            for k in rng.choice(len(rows), size=args.fd_blocks, replace=False):
+            for n, k in enumerate(picks[args.fd_first:args.fd_first + args.fd_blocks]):
                 d = torch.zeros_like(hv)
-                v = rng.normal(size=int(sz[k]))
+                v = np.random.default_rng((7, int(k))).normal(size=int(sz[k]))
                 d[offs[k]:offs[k + 1]] = torch.from_numpy(v / np.linalg.norm(v)).to(d.device)
                 lp = AG.purification_loss(hv + 1e-6 * d, ph, atoms.blk, lo, hi, steps).item()
                 lm = AG.purification_loss(hv - 1e-6 * d, ph, atoms.blk, lo, hi, steps).item()
                 an = float(torch.dot(grads["fp64"], d))
-                worst = max(worst, abs((lp - lm) / 2e-6 - an) / max(abs(an), 1.0))
+                e = abs((lp - lm) / 2e-6 - an) / max(abs(an), 1.0)
+                worst = max(worst, e)
+                print(f"fd block {args.fd_first + n} (block {int(k)}): dL.d {an:.9e}, "
+                      f"scaled error {e:.2e}", file=sys.stderr, flush=True)
         fd = worst
     dd.close()
     ctx.close()
@@ -273,7 +327,8 @@ def run_config5(args):
                                    f"L = tr(P_{steps} H), {steps} McWeeny steps, checkpoint "
                                    f"every 2", **res,
                        "grad_rel_diff_bf16_vs_fp64": gdiff,
-                       "fd_blocks": args.fd_blocks, "fd_worst_scaled_error": fd},
+                       "fd_blocks": args.fd_blocks, "fd_first": args.fd_first,
+                       "fd_worst_scaled_error": fd},
                 higher_is_better=False)
 
 

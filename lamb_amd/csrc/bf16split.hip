@@ -275,7 +275,7 @@ __global__ __launch_bounds__(256, 2) void numeric_bf16_kernel(
       const int c0 = base + incl - nch;
       const uint32_t m = (uint32_t)en.rmask | ((uint32_t)en.cmask << 16);
       const uint32_t g = entry_groups(sm, en.rmask, en.cmask);
-      const uint32_t ac = en.a_col / kChunk, bc = en.b_col / kChunk;
+      const uint32_t ac = (en.a_col & ~kEntryPartial) / kChunk, bc = en.b_col / kChunk;
       for (int c = 0; c < nch; ++c) {
         tbl_ab[c0 + c] = make_uint2(ac + c, bc + c);
         tbl_m[c0 + c] = m;

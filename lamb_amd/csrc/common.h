@@ -27,6 +27,12 @@ constexpr int kPitch = kBandRows + 1;
 // [kstart, next entry's kstart) of its tile (the tile's ktot for the last).
 // rmask / cmask select the atoms of the row / column band whose blocks take
 // part (bit = position of the atom inside its band).
+// a_col: A panel column of the entry's first K column; bit 31 (kEntryPartial)
+// is set when the prefilter or an output mask cut the entry's row / column
+// atoms below its strips' masks. Only such entries need per-lane masking:
+// the panels already hold zeros outside the strips' atoms.
+constexpr uint32_t kEntryPartial = 0x80000000u;
+
 struct Entry {
   uint32_t a_col;
   uint32_t b_col;

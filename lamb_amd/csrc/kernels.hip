@@ -1,4 +1,6 @@
 // HIP kernels for gfx950: block norms, FP64 numeric phase, trace.
+#include <type_traits>
+
 #include "common.h"
 #include "tile_common.h"
 
@@ -133,16 +135,18 @@ constexpr int kPrefetch = LAMB_NUMERIC_PREFETCH;  // K steps in flight per wave
 // The step table of the K window [Q0, Q0 + kTableCols) of a tile: per K
 // column its A and B panel columns (the zero columns past the end) and atom
 // masks, per step of 4 columns the live row / column 4-groups. One thread
-// per entry writes its columns. Returns the window's step count; the table
-// is visible to all threads on return.
+// per entry writes its columns; *partial becomes 1 when an entry of the
+// window is partial (its atoms cut below its strips'). Returns the window's
+// step count; the table and *partial are visible to all threads on return.
 __device__ __forceinline__ int build_step_table(const TileInfo& T, const TileSmem& sm,
                                                 const Entry* __restrict__ entries, int Q0,
                                                 uint2* col_ab, uint32_t* col_m, uint32_t* step_g,
-                                                uint32_t a_zero, uint32_t b_zero) {
+                                                uint32_t a_zero, uint32_t b_zero, int* partial) {
   const int tid = threadIdx.x;
   const int ncols = min(kTableCols, T.ktot - Q0);
   const int nsteps = (ncols + 3) >> 2;
   __syncthreads();  // the previous window is done with the table
+  if (tid == 0) *partial = 0;
   for (int t = tid; t < 4 * nsteps; t += 256) {
     col_ab[t] = make_uint2(a_zero, b_zero);
     col_m[t] = 0u;
@@ -155,11 +159,13 @@ __device__ __forceinline__ int build_step_table(const TileInfo& T, const TileSme
     const int ke = (e + 1 < T.ne) ? entries[T.e0 + e + 1].kstart : T.ktot;
     const int lo = max(ks, Q0), hi = min(ke, Q0 + ncols);
     if (lo >= hi) continue;
+    if (en.a_col & kEntryPartial) *partial = 1;
+    const uint32_t ac = en.a_col & ~kEntryPartial;
     const uint32_t m = (uint32_t)en.rmask | ((uint32_t)en.cmask << 16);
     const uint32_t g = entry_groups(sm, en.rmask, en.cmask);
     for (int Q = lo; Q < hi; ++Q) {
       const int q = Q - Q0;
-      col_ab[q] = make_uint2(en.a_col + (uint32_t)(Q - ks), en.b_col + (uint32_t)(Q - ks));
+      col_ab[q] = make_uint2(ac + (uint32_t)(Q - ks), en.b_col + (uint32_t)(Q - ks));
       col_m[q] = m;
     }
     for (int st = (lo - Q0) >> 2; st <= (hi - 1 - Q0) >> 2; ++st) atomicOr(&step_g[st], g);
@@ -205,10 +211,13 @@ __global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
 #pragma unroll
   for (int x = 0; x < 4; ++x) acc[x] = dbl4{0.0, 0.0, 0.0, 0.0};
 
-  for (int Q0 = 0; Q0 < ktot; Q0 += kTableCols) {
-    const int nsteps = build_step_table(T, sm, entries, Q0, col_ab, col_m, step_g, a_zero, b_zero);
-
-    // ---- per-wave walk over the window's live steps, kPrefetch ahead ----
+  __shared__ int win_partial;
+  // the walk over one window's live steps, kPrefetch ahead; masked: lanes
+  // test their atom against each K column's entry masks (windows holding a
+  // partial entry), else whole panel columns are loaded (the panels are zero
+  // outside the strips' atoms, which then equal the entries' atoms)
+  auto walk = [&](int nsteps, auto masked_tag) {
+    constexpr bool kMasked = decltype(masked_tag)::value;
     int cbase = -64;        // first step of the current chunk
     uint64_t live = 0;      // live steps of the chunk not yet taken
     uint32_t chunk_bm = 0;  // lane l: block mask of step cbase + l
@@ -233,16 +242,26 @@ __global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
       blkmask[S] = bm;                                                             \
       const int Q = 4 * st + kc;                                                   \
       const uint2 ab = col_ab[Q];                                                  \
-      const uint32_t cm = col_m[Q];                                                \
-      _Pragma("unroll") for (int h = 0; h < 2; ++h) {                              \
-        const uint32_t need = (bm & (3u << (2 * h))) ? rbit[h] : 0u;               \
-        const uint32_t c = (cm & need) ? ab.x : a_zero;                            \
-        ra[S][h] = a_lane[(int64_t)c * 64 + 16 * h];                               \
-      }                                                                            \
-      _Pragma("unroll") for (int q = 0; q < 2; ++q) {                              \
-        const uint32_t need = (bm & (5u << q)) ? cbit[q] : 0u;                     \
-        const uint32_t c = (cm & need) ? ab.y : b_zero;                            \
-        rb[S][q] = b_lane[(int64_t)c * 64 + 16 * q];                               \
+      if constexpr (kMasked) {                                                     \
+        const uint32_t cm = col_m[Q];                                              \
+        _Pragma("unroll") for (int h = 0; h < 2; ++h) {                            \
+          const uint32_t need = (bm & (3u << (2 * h))) ? rbit[h] : 0u;             \
+          const uint32_t c = (cm & need) ? ab.x : a_zero;                          \
+          ra[S][h] = a_lane[(int64_t)c * 64 + 16 * h];                             \
+        }                                                                          \
+        _Pragma("unroll") for (int q = 0; q < 2; ++q) {                            \
+          const uint32_t need = (bm & (5u << q)) ? cbit[q] : 0u;                   \
+          const uint32_t c = (cm & need) ? ab.y : b_zero;                          \
+          rb[S][q] = b_lane[(int64_t)c * 64 + 16 * q];                             \
+        }                                                                          \
+      } else {                                                                     \
+        /* past the last live step the loads read step 0 and feed no MFMA */       \
+        const double* pa = a_lane + (int64_t)ab.x * 64;                            \
+        const double* pb = b_lane + (int64_t)ab.y * 64;                            \
+        ra[S][0] = pa[0];                                                          \
+        ra[S][1] = pa[16];                                                         \
+        rb[S][0] = pb[0];                                                          \
+        rb[S][1] = pb[16];                                                         \
       }                                                                            \
     }
 #define LAMB_MFMA(S)                                                               \
@@ -272,6 +291,15 @@ __global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
 #undef LAMB_SLOTS
 #undef LAMB_FETCH
 #undef LAMB_MFMA
+  };
+
+  for (int Q0 = 0; Q0 < ktot; Q0 += kTableCols) {
+    const int nsteps = build_step_table(T, sm, entries, Q0, col_ab, col_m, step_g, a_zero, b_zero,
+                                        &win_partial);
+    if (win_partial)
+      walk(nsteps, std::true_type{});
+    else
+      walk(nsteps, std::false_type{});
   }
   __syncthreads();
 
@@ -337,6 +365,7 @@ __global__ __launch_bounds__(256, 2) void numeric_f64_ksplit_kernel(
   uint32_t* col_m = reinterpret_cast<uint32_t*>(col_ab + kTableCols);
   uint32_t* step_g = col_m + kTableCols;
   __shared__ TileSmem sm;
+  __shared__ int win_partial;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -366,7 +395,8 @@ __global__ __launch_bounds__(256, 2) void numeric_f64_ksplit_kernel(
   uint32_t seen = 0;  // live steps met so far (wave-uniform)
 
   for (int Q0 = 0; Q0 < T.ktot; Q0 += kTableCols) {
-    const int nsteps = build_step_table(T, sm, entries, Q0, col_ab, col_m, step_g, a_zero, b_zero);
+    const int nsteps = build_step_table(T, sm, entries, Q0, col_ab, col_m, step_g, a_zero, b_zero,
+                                        &win_partial);
     int cbase = -64;
     uint64_t live = 0;
     uint32_t chunk_bm = 0;

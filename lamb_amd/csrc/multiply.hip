@@ -201,9 +201,10 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
   // ---- strips and packed panels ----
   HIPCHK(ctx, build_strips(ctx, S.W, MA, D, true, S.sa), "A strips");
   HIPCHK(ctx, build_strips(ctx, S.W, MB, D, false, S.sb), "B strips");
-  // panel columns are addressed by 32-bit indices; one zero column follows
-  // the last panel of each side
-  if (S.sa.pk_total / 64 >= (int64_t)UINT32_MAX || S.sb.pk_total / 64 >= (int64_t)UINT32_MAX ||
+  // panel columns are addressed by 32-bit indices (31 bits on the A side,
+  // whose top bit flags partial entries); one zero column follows the last
+  // panel of each side
+  if (S.sa.pk_total / 64 >= (int64_t)INT32_MAX || S.sb.pk_total / 64 >= (int64_t)UINT32_MAX ||
       (int64_t)slices * (S.sa.pk_total / 512 + 1) * 64 >= (int64_t)INT32_MAX * 64)
     return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_OVERFLOW_ERROR,
                      "multiply: packed panels exceed 32-bit column indices");

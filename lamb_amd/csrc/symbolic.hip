@@ -484,7 +484,8 @@ __global__ __launch_bounds__(64) void sym_fill(
         uint32_t pairs[kBandAtoms];
         if (!mask_entry(mpat, nbands, I, J, rm, cm, pairs)) return;
         Entry en;
-        en.a_col = (uint32_t)(a_pk[sa] / 64);
+        en.a_col = (uint32_t)(a_pk[sa] / 64) |
+                   ((rm == SA->mask && cm == SB->mask) ? 0u : kEntryPartial);
         en.b_col = (uint32_t)(b_pk[sb] / 64);
         en.kstart = kk;
         en.rmask = (uint16_t)rm;

@@ -17,8 +17,9 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-def gpu_multiply(ctx, atoms, a, b, *, eps=0.0, alpha=1.0, beta=0.0, c_old=None, spatial=True):
-    dist = Distribution(atoms.blk, atoms.pos, atoms.box)
+def gpu_multiply(ctx, atoms, a, b, *, eps=0.0, alpha=1.0, beta=0.0, c_old=None, spatial=True,
+                 n_procs=1):
+    dist = Distribution(atoms.blk, atoms.pos, atoms.box, n_procs=n_procs)
     dd = D.DeviceDistribution(ctx, dist, spatial=spatial)
     da = D.DeviceMatrix.from_lamb(dd, LambMatrix.from_global(dist, a))
     db = D.DeviceMatrix.from_lamb(dd, LambMatrix.from_global(dist, b))
@@ -33,11 +34,11 @@ def gpu_multiply(ctx, atoms, a, b, *, eps=0.0, alpha=1.0, beta=0.0, c_old=None, 
 
 
 def check(ctx, atoms, a, b, *, eps=0.0, alpha=1.0, beta=0.0, c_old=None, spatial=True,
-          exact=False):
-    print(f"case: atoms={atoms.n_atoms} blocks={a.n_blocks} eps={eps} alpha={alpha} beta={beta}",
-          flush=True)
+          exact=False, n_procs=1):
+    print(f"case: atoms={atoms.n_atoms} blocks={a.n_blocks} eps={eps} alpha={alpha} beta={beta}"
+          f" n_procs={n_procs}", flush=True)
     got, st = gpu_multiply(ctx, atoms, a, b, eps=eps, alpha=alpha, beta=beta, c_old=c_old,
-                           spatial=spatial)
+                           spatial=spatial, n_procs=n_procs)
     ref, rst = oracle.multiply(a, b, eps=eps, alpha=alpha, beta=beta, c_old=c_old)
     same, err = compare(got, ref)
     assert same, "stored block pattern differs from the oracle"
@@ -63,6 +64,17 @@ def test_water_eps(ctx):
     a = S.decay_matrix(atoms, 8.0, 0.4, 12)
     b = S.decay_matrix(atoms, 8.0, 0.4, 13)
     check(ctx, atoms, a, b, eps=1e-7)
+
+
+@pytest.mark.parametrize("n_procs", [2, 8])
+def test_several_subsets(ctx, n_procs):
+    """A distribution with several spatial subsets (one process owning every
+    sector), as used for batches beyond the int16 local index of one subset."""
+    atoms = S.water_system(300, seed=41)
+    a = S.decay_matrix(atoms, 7.0, 0.4, 42)
+    b = S.decay_matrix(atoms, 7.0, 0.4, 43)
+    assert Distribution(atoms.blk, atoms.pos, atoms.box, n_procs=n_procs).n_subsets > 1
+    check(ctx, atoms, a, b, eps=1e-6, n_procs=n_procs)
 
 
 def test_three_kinds(ctx):
