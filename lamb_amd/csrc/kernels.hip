@@ -131,6 +131,11 @@ constexpr int kTableCols = 2048;
 #define LAMB_NUMERIC_WAVES 4
 #endif
 constexpr int kPrefetch = LAMB_NUMERIC_PREFETCH;  // K steps in flight per wave
+#ifndef LAMB_F64_HALF_SELECT
+#define LAMB_F64_HALF_SELECT 0
+#endif
+// unmasked walk: halves / quads without a live block read the zero column
+constexpr bool kHalfSelect = LAMB_F64_HALF_SELECT != 0;
 
 // The step table of the K window [Q0, Q0 + kTableCols) of a tile: per K
 // column its A and B panel columns (the zero columns past the end) and atom
@@ -254,6 +259,12 @@ __global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
           const uint32_t c = (cm & need) ? ab.y : b_zero;                          \
           rb[S][q] = b_lane[(int64_t)c * 64 + 16 * q];                             \
         }                                                                          \
+      } else if constexpr (kHalfSelect) {                                         \
+        /* halves / quads without a live block read the zero column */             \
+        ra[S][0] = a_lane[(int64_t)((bm & 3u) ? ab.x : a_zero) * 64];              \
+        ra[S][1] = a_lane[(int64_t)((bm & 12u) ? ab.x : a_zero) * 64 + 16];        \
+        rb[S][0] = b_lane[(int64_t)((bm & 5u) ? ab.y : b_zero) * 64];              \
+        rb[S][1] = b_lane[(int64_t)((bm & 10u) ? ab.y : b_zero) * 64 + 16];        \
       } else {                                                                     \
         /* past the last live step the loads read step 0 and feed no MFMA */       \
         const double* pa = a_lane + (int64_t)ab.x * 64;                            \
