@@ -54,26 +54,43 @@ def parse():
 
 
 def cpu_baseline(a, b, eps, every):
-    """Time the CPU FP64 oracle on a row sample of the same product."""
+    """Time the CPU FP64 oracle on a row sample of the same product, on all
+    host threads and (on a 16x smaller sample) on one."""
     import oracle
     from lamb_amd.systems import BlockCSR
-    rows = np.arange(a.n_atoms)
-    keep = (rows % every) == 0
-    cnt = np.where(keep, np.diff(a.row_p), 0)
-    row_p = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int64)
-    sel = np.repeat(keep, np.diff(a.row_p))
-    sub = BlockCSR(a.blk, row_p, a.col[sel], a.blk_off[sel], a.data)
+
+    def sample(step):
+        rows = np.arange(a.n_atoms)
+        keep = (rows % step) == 0
+        cnt = np.where(keep, np.diff(a.row_p), 0)
+        row_p = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int64)
+        sel = np.repeat(keep, np.diff(a.row_p))
+        return BlockCSR(a.blk, row_p, a.col[sel], a.blk_off[sel], a.data), int(keep.sum())
+
     oracle.lib()
+    cores = oracle.num_threads()
+    sub, n_rows = sample(every)
     t0 = time.perf_counter()
     c, st = oracle.multiply(sub, b, eps=eps)
     dt = time.perf_counter() - t0
+    sub1, n_rows1 = sample(16 * every)
+    oracle.set_threads(1)
+    try:
+        t1 = time.perf_counter()
+        _, st1 = oracle.multiply(sub1, b, eps=eps)
+        dt1 = time.perf_counter() - t1
+    finally:
+        oracle.set_threads(cores)
     return {
         "value": st["flops"] / dt / 1e9,
         "unit": "GFLOP/s",
-        "cores": oracle.num_threads(),
+        "cores": cores,
         "kind": "port",
-        "sample": f"every {every}th block row of A ({int(keep.sum())} of {a.n_atoms} rows), "
+        "sample": f"every {every}th block row of A ({n_rows} of {a.n_atoms} rows), "
                   f"{st['flops']:.3e} flop in {dt:.2f} s, FP64 C oracle, OpenMP",
+        "value_1_core": st1["flops"] / dt1 / 1e9,
+        "sample_1_core": f"every {16 * every}th block row of A ({n_rows1} rows), "
+                         f"{st1['flops']:.3e} flop in {dt1:.2f} s on one thread",
     }
 
 

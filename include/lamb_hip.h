@@ -239,6 +239,51 @@ int32_t lamb_hip_matrix_release_values(lamb_hip_mat* X);
 /* Global (0-based) row and column atom of each stored block, canonical order. */
 int32_t lamb_hip_matrix_block_atoms(lamb_hip_mat* X, int32_t* rows, int32_t* cols);
 
+/* --- DBCSR files ---------------------------------------------------------
+ * The dump format that lamb_load_dbcsr reads (src/lamb_io_operations.F90:
+ * 296-330; chunk offsets :557-597), in native byte order:
+ *   sub-header1: version char[10], name char[80], matrix type char,
+ *                int32 n_chunks, data_type, block rows, block columns,
+ *                int32 row block sizes[block rows], column block sizes;
+ *   one sub-header2 per chunk: int32 nblks, data area size (FP64 values);
+ *   one sub-header3 per chunk: int32 row_p[block rows + 1] (0-based
+ *                exclusive offsets), col_i[nblks] (1-based block column),
+ *                blk_p[nblks] (1-based start in the chunk's data area);
+ *   one FP64 data area per chunk.
+ * The two readers need no device and no context: on failure they return a
+ * status and, when msg is not NULL, a NUL-terminated message. */
+typedef struct lamb_hip_dbcsr_info {
+  char version[11];     /* NUL-terminated */
+  char name[81];        /* NUL-terminated, trailing blanks kept */
+  char matrix_type;     /* 'N' (no symmetry), 'S', 'A', 'H' or 'K' */
+  int32_t n_chunks;     /* numnodes of the writer */
+  int32_t data_type;
+  int32_t n_block_rows, n_block_cols;
+  int64_t n_blocks;     /* over all chunks */
+  int64_t n_values;     /* sum of m*n over all blocks */
+} lamb_hip_dbcsr_info;
+/* Sub-headers 1 and 2 of every chunk (lamb_load_dbcsr :353-384 and
+ * load_dbcsr_chunks :568-585, with the reference's checks: block sizes
+ * below 127, every block inside its chunk's data area). row_blk_size /
+ * col_blk_size (int32[n_block_rows] / [n_block_cols]) may be NULL. */
+int32_t lamb_hip_dbcsr_read_info(const char* path, lamb_hip_dbcsr_info* info,
+                                 int32_t* row_blk_size, int32_t* col_blk_size, char* msg,
+                                 int32_t msg_len);
+/* Every block of every chunk, in file order (chunk, then row, then the
+ * chunk's col_i order): rows / cols int32[n_blocks] 1-based block row and
+ * column; values FP64[n_values] the blocks back to back, each column-major
+ * (the coordinate index load_dbcsr_chunks builds, :642-656). */
+int32_t lamb_hip_dbcsr_read_blocks(const char* path, int64_t n_blocks, int32_t* rows,
+                                   int32_t* cols, int64_t n_values, double* values, char* msg,
+                                   int32_t msg_len);
+/* load_dbcsr_chunks into a device matrix (:520-740): each chunk's blocks are
+ * staged as one data store, like lamb_hip_distribute, with (subset, local)
+ * taken from mat's distribution, and the matrix is committed. The file must
+ * be of type 'N' (the reference's load rejects the symmetric types as
+ * unimplemented, :409-412) and square over the distribution's atoms, with
+ * its block sizes. */
+int32_t lamb_hip_dbcsr_load(lamb_hip_mat* mat, const char* path);
+
 #ifdef __cplusplus
 }
 #endif

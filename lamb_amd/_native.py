@@ -40,6 +40,21 @@ class Stats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class DbcsrInfo(C.Structure):
+    """lamb_hip_dbcsr_info."""
+    _fields_ = [
+        ("version", C.c_char * 11),
+        ("name", C.c_char * 81),
+        ("matrix_type", C.c_char),
+        ("n_chunks", C.c_int32),
+        ("data_type", C.c_int32),
+        ("n_block_rows", C.c_int32),
+        ("n_block_cols", C.c_int32),
+        ("n_blocks", C.c_int64),
+        ("n_values", C.c_int64),
+    ]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "lamb_hip_init": (C.c_int32, [C.c_int32, vpp]),
@@ -88,6 +103,11 @@ SIGNATURES = {
     "lamb_hip_matrix_set_values": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64]),
     "lamb_hip_matrix_release_values": (C.c_int32, [C.c_void_p]),
     "lamb_hip_matrix_block_atoms": (C.c_int32, [C.c_void_p, i32p, i32p]),
+    "lamb_hip_dbcsr_read_info": (C.c_int32, [C.c_char_p, C.c_void_p, i32p, i32p, C.c_char_p,
+                                             C.c_int32]),
+    "lamb_hip_dbcsr_read_blocks": (C.c_int32, [C.c_char_p, C.c_int64, i32p, i32p, C.c_int64,
+                                               f64p, C.c_char_p, C.c_int32]),
+    "lamb_hip_dbcsr_load": (C.c_int32, [C.c_void_p, C.c_char_p]),
 }
 
 _lib = None

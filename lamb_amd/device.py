@@ -2,6 +2,7 @@
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -105,6 +106,18 @@ class DeviceMatrix:
     def from_lamb(cls, ddist: DeviceDistribution, m: LambMatrix) -> "DeviceMatrix":
         dm = cls(ddist)
         dm.upload(m)
+        return dm
+
+    @classmethod
+    def from_dbcsr(cls, ddist: DeviceDistribution, path) -> "DeviceMatrix":
+        """A DBCSR dump loaded into the device pool (lamb_hip_dbcsr_load)."""
+        dm = cls(ddist)
+        try:
+            dm.ctx.check(dm.ctx.lib.lamb_hip_dbcsr_load(dm.h, os.fsencode(os.fspath(path))),
+                         "dbcsr_load")
+        except Exception:
+            dm.close()
+            raise
         return dm
 
     def upload(self, m: LambMatrix):

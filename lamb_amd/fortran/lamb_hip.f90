@@ -50,8 +50,8 @@ module lamb_hip
   public :: lamb_hip_matrix_create, lamb_hip_matrix_destroy, lamb_hip_matrix_copy
   public :: lamb_hip_sector_upload, lamb_hip_distribute, lamb_hip_matrix_commit
   public :: lamb_hip_sector_query, lamb_hip_sector_download, lamb_hip_matrix_info
-  public :: lamb_hip_multiply, lamb_hip_trace
-  public :: lamb_multiply_hip, internal_distribute_hip
+  public :: lamb_hip_multiply, lamb_hip_trace, lamb_hip_dbcsr_load
+  public :: lamb_multiply_hip, internal_distribute_hip, lamb_load_dbcsr_hip
 
   interface
     function lamb_hip_init(device, ctx) bind(C, name="lamb_hip_init") result(st)
@@ -195,6 +195,13 @@ module lamb_hip
       real(c_double), intent(out) :: trace
       integer(c_int32_t) :: st
     end function
+
+    function lamb_hip_dbcsr_load(mat, path) bind(C, name="lamb_hip_dbcsr_load") result(st)
+      import :: c_int32_t, c_ptr, c_char
+      type(c_ptr), value :: mat
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t) :: st
+    end function
   end interface
 
 contains
@@ -250,6 +257,18 @@ contains
     st = lamb_hip_distribute(matrix%mat, n, subset_idx, local_idx, blkdata_extents, &
                              data_area, int(size(data_area), c_int64_t))
     if (st == lamb_hip_ok) st = lamb_hip_matrix_commit(matrix%mat)
+    call fetch_error(matrix%ctx, st, error)
+  end subroutine
+
+  ! A DBCSR dump into the device matrix, in lamb_load_dbcsr's argument order
+  ! (src/lamb_io_operations.F90:267-271) without the MPI environment: the
+  ! matrix already exists over a distribution of the file's atoms.
+  subroutine lamb_load_dbcsr_hip(matrix, filename, error)
+    type(lamb_hip_matrix), intent(inout) :: matrix
+    character(len=*), intent(in) :: filename
+    type(lamb_hip_error), intent(inout) :: error
+    integer(c_int32_t) :: st
+    st = lamb_hip_dbcsr_load(matrix%mat, trim(filename)//c_null_char)
     call fetch_error(matrix%ctx, st, error)
   end subroutine
 
