@@ -239,6 +239,20 @@ int32_t lamb_hip_matrix_release_values(lamb_hip_mat* X);
 /* Global (0-based) row and column atom of each stored block, canonical order. */
 int32_t lamb_hip_matrix_block_atoms(lamb_hip_mat* X, int32_t* rows, int32_t* cols);
 
+/* --- atom partition on the device ---------------------------------------
+ * The atom loop of lamb_set_partition_update (src/lamb_set_partition_methods.
+ * F90:98-120) for atoms that move: subset_map(g) = 1 + cx + cy*nx +
+ * cz*nx*ny with c_d = mod(int(x_d * mult_d), n_d), mult_d = n_d / box_d in
+ * FP32 (lamb_partition_which_cube1, src/lamb_partition_methods.F90:85-107),
+ * and order_map(g) the arrival order of g within its subset. coords
+ * (FP32 xyz per atom, wrapped into the box) and the outputs are host arrays;
+ * dims are the cube counts per axis (at most 4096 cubes). Atoms outside the
+ * cubes are an error (wrong args), as is a subset beyond the int16 local
+ * index (overflow). */
+int32_t lamb_hip_partition_atoms(lamb_hip_ctx* ctx, int64_t n_atoms, const float* coords,
+                                 const float* box, const int32_t* dims, int32_t* subset_map,
+                                 int16_t* order_map);
+
 /* --- DBCSR files ---------------------------------------------------------
  * The dump format that lamb_load_dbcsr reads (src/lamb_io_operations.F90:
  * 296-330; chunk offsets :557-597), in native byte order:

@@ -131,28 +131,32 @@ constexpr int kTableCols = 2048;
 #define LAMB_NUMERIC_WAVES 4
 #endif
 constexpr int kPrefetch = LAMB_NUMERIC_PREFETCH;  // K steps in flight per wave
-#ifndef LAMB_F64_HALF_SELECT
-#define LAMB_F64_HALF_SELECT 0
+#ifndef LAMB_F64_STEP_COLS
+#define LAMB_F64_STEP_COLS 4
 #endif
-// unmasked walk: halves / quads without a live block read the zero column
-constexpr bool kHalfSelect = LAMB_F64_HALF_SELECT != 0;
+// K columns per step of the quadrant kernel: one or two groups of the
+// 4 columns a v_mfma_f64_16x16x4 consumes
+constexpr int kStepCols = LAMB_F64_STEP_COLS;
+constexpr int kGroups = kStepCols / 4;
 
 // The step table of the K window [Q0, Q0 + kTableCols) of a tile: per K
 // column its A and B panel columns (the zero columns past the end) and atom
-// masks, per step of 4 columns the live row / column 4-groups. One thread
+// masks, per step of KS columns the live row / column 4-groups. One thread
 // per entry writes its columns; *partial becomes 1 when an entry of the
 // window is partial (its atoms cut below its strips'). Returns the window's
 // step count; the table and *partial are visible to all threads on return.
+template <int KS>
 __device__ __forceinline__ int build_step_table(const TileInfo& T, const TileSmem& sm,
                                                 const Entry* __restrict__ entries, int Q0,
                                                 uint2* col_ab, uint32_t* col_m, uint32_t* step_g,
                                                 uint32_t a_zero, uint32_t b_zero, int* partial) {
+  static_assert(KS == 4 || KS == 8, "K columns per step");
   const int tid = threadIdx.x;
   const int ncols = min(kTableCols, T.ktot - Q0);
-  const int nsteps = (ncols + 3) >> 2;
+  const int nsteps = (ncols + KS - 1) / KS;
   __syncthreads();  // the previous window is done with the table
   if (tid == 0) *partial = 0;
-  for (int t = tid; t < 4 * nsteps; t += 256) {
+  for (int t = tid; t < KS * nsteps; t += 256) {
     col_ab[t] = make_uint2(a_zero, b_zero);
     col_m[t] = 0u;
   }
@@ -173,7 +177,7 @@ __device__ __forceinline__ int build_step_table(const TileInfo& T, const TileSme
       col_ab[q] = make_uint2(ac + (uint32_t)(Q - ks), en.b_col + (uint32_t)(Q - ks));
       col_m[q] = m;
     }
-    for (int st = (lo - Q0) >> 2; st <= (hi - 1 - Q0) >> 2; ++st) atomicOr(&step_g[st], g);
+    for (int st = (lo - Q0) / KS; st <= (hi - 1 - Q0) / KS; ++st) atomicOr(&step_g[st], g);
   }
   __syncthreads();
   return nsteps;
@@ -226,7 +230,9 @@ __global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
     int cbase = -64;        // first step of the current chunk
     uint64_t live = 0;      // live steps of the chunk not yet taken
     uint32_t chunk_bm = 0;  // lane l: block mask of step cbase + l
-    double ra[8][2], rb[8][2];  // slots 0..kPrefetch-1 used
+    // slot S holds, per group g of 4 K columns, A halves [2g], [2g+1] and
+    // B quads [2g], [2g+1] (slots 0..kPrefetch-1 used)
+    double ra[8][2 * kGroups], rb[8][2 * kGroups];
     uint32_t blkmask[8];
 #define LAMB_FETCH(S)                                                              \
     {                                                                              \
@@ -245,43 +251,42 @@ __global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
         bm = __builtin_amdgcn_readlane(chunk_bm, l);                               \
       }                                                                            \
       blkmask[S] = bm;                                                             \
-      const int Q = 4 * st + kc;                                                   \
-      const uint2 ab = col_ab[Q];                                                  \
-      if constexpr (kMasked) {                                                     \
-        const uint32_t cm = col_m[Q];                                              \
-        _Pragma("unroll") for (int h = 0; h < 2; ++h) {                            \
-          const uint32_t need = (bm & (3u << (2 * h))) ? rbit[h] : 0u;             \
-          const uint32_t c = (cm & need) ? ab.x : a_zero;                          \
-          ra[S][h] = a_lane[(int64_t)c * 64 + 16 * h];                             \
+      _Pragma("unroll") for (int g = 0; g < kGroups; ++g) {                        \
+        const int Q = kStepCols * st + 4 * g + kc;                                 \
+        const uint2 ab = col_ab[Q];                                                \
+        if constexpr (kMasked) {                                                   \
+          const uint32_t cm = col_m[Q];                                            \
+          _Pragma("unroll") for (int h = 0; h < 2; ++h) {                          \
+            const uint32_t need = (bm & (3u << (2 * h))) ? rbit[h] : 0u;           \
+            const uint32_t c = (cm & need) ? ab.x : a_zero;                        \
+            ra[S][2 * g + h] = a_lane[(int64_t)c * 64 + 16 * h];                   \
+          }                                                                        \
+          _Pragma("unroll") for (int q = 0; q < 2; ++q) {                          \
+            const uint32_t need = (bm & (5u << q)) ? cbit[q] : 0u;                 \
+            const uint32_t c = (cm & need) ? ab.y : b_zero;                        \
+            rb[S][2 * g + q] = b_lane[(int64_t)c * 64 + 16 * q];                   \
+          }                                                                        \
+        } else {                                                                   \
+          /* past the last live step the loads read step 0 and feed no MFMA */     \
+          const double* pa = a_lane + (int64_t)ab.x * 64;                          \
+          const double* pb = b_lane + (int64_t)ab.y * 64;                          \
+          ra[S][2 * g] = pa[0];                                                    \
+          ra[S][2 * g + 1] = pa[16];                                               \
+          rb[S][2 * g] = pb[0];                                                    \
+          rb[S][2 * g + 1] = pb[16];                                               \
         }                                                                          \
-        _Pragma("unroll") for (int q = 0; q < 2; ++q) {                            \
-          const uint32_t need = (bm & (5u << q)) ? cbit[q] : 0u;                   \
-          const uint32_t c = (cm & need) ? ab.y : b_zero;                          \
-          rb[S][q] = b_lane[(int64_t)c * 64 + 16 * q];                             \
-        }                                                                          \
-      } else if constexpr (kHalfSelect) {                                         \
-        /* halves / quads without a live block read the zero column */             \
-        ra[S][0] = a_lane[(int64_t)((bm & 3u) ? ab.x : a_zero) * 64];              \
-        ra[S][1] = a_lane[(int64_t)((bm & 12u) ? ab.x : a_zero) * 64 + 16];        \
-        rb[S][0] = b_lane[(int64_t)((bm & 5u) ? ab.y : b_zero) * 64];              \
-        rb[S][1] = b_lane[(int64_t)((bm & 10u) ? ab.y : b_zero) * 64 + 16];        \
-      } else {                                                                     \
-        /* past the last live step the loads read step 0 and feed no MFMA */       \
-        const double* pa = a_lane + (int64_t)ab.x * 64;                            \
-        const double* pb = b_lane + (int64_t)ab.y * 64;                            \
-        ra[S][0] = pa[0];                                                          \
-        ra[S][1] = pa[16];                                                         \
-        rb[S][0] = pb[0];                                                          \
-        rb[S][1] = pb[16];                                                         \
       }                                                                            \
     }
+    // the groups' MFMAs interleave over the four accumulators
 #define LAMB_MFMA(S)                                                               \
     {                                                                              \
       const uint32_t bm = blkmask[S];                                              \
-      _Pragma("unroll") for (int h = 0; h < 2; ++h)                                \
-        _Pragma("unroll") for (int q = 0; q < 2; ++q)                              \
-          if ((bm >> (2 * h + q)) & 1u)                                            \
-            acc[2 * h + q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[S][h], rb[S][q], acc[2 * h + q], 0, 0, 0); \
+      _Pragma("unroll") for (int g = 0; g < kGroups; ++g)                          \
+        _Pragma("unroll") for (int h = 0; h < 2; ++h)                              \
+          _Pragma("unroll") for (int q = 0; q < 2; ++q)                            \
+            if ((bm >> (2 * h + q)) & 1u)                                          \
+              acc[2 * h + q] = __builtin_amdgcn_mfma_f64_16x16x4f64(               \
+                  ra[S][2 * g + h], rb[S][2 * g + q], acc[2 * h + q], 0, 0, 0);    \
     }
     // slots spelled out so that every register array index is a constant
 #define LAMB_SLOTS(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
@@ -305,7 +310,8 @@ __global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
   };
 
   for (int Q0 = 0; Q0 < ktot; Q0 += kTableCols) {
-    const int nsteps = build_step_table(T, sm, entries, Q0, col_ab, col_m, step_g, a_zero, b_zero,
+    const int nsteps = build_step_table<kStepCols>(T, sm, entries, Q0, col_ab, col_m, step_g,
+                                                   a_zero, b_zero,
                                         &win_partial);
     if (win_partial)
       walk(nsteps, std::true_type{});
@@ -406,7 +412,7 @@ __global__ __launch_bounds__(256, 2) void numeric_f64_ksplit_kernel(
   uint32_t seen = 0;  // live steps met so far (wave-uniform)
 
   for (int Q0 = 0; Q0 < T.ktot; Q0 += kTableCols) {
-    const int nsteps = build_step_table(T, sm, entries, Q0, col_ab, col_m, step_g, a_zero, b_zero,
+    const int nsteps = build_step_table<4>(T, sm, entries, Q0, col_ab, col_m, step_g, a_zero, b_zero,
                                         &win_partial);
     int cbase = -64;
     uint64_t live = 0;

@@ -296,8 +296,11 @@ __global__ __launch_bounds__(64) void sym_count(
         for (int q = 0; q < kBandAtoms; ++q)
           if ((cm >> q) & 1u) cr += bands[J].rs[q + 1] - bands[J].rs[q];
         flops += 2.0 * rows_of_mask(rsI, rm) * kd * cr;
-        for (int p = 0; p < kBandAtoms; ++p)
-          if (pairs[p]) atomicOr(&pt[(int64_t)J * (kBandAtoms / 2) + (p >> 1)], pairs[p] << (16 * (p & 1)));
+        // one atomic per pattern word (two row atoms)
+        for (int wd = 0; wd < kBandAtoms / 2; ++wd) {
+          const uint32_t v = pairs[2 * wd] | (pairs[2 * wd + 1] << 16);
+          if (v) atomicOr(&pt[(int64_t)J * (kBandAtoms / 2) + wd], v);
+        }
       });
       if (n) {
         atomicAdd(&c[J], n);
@@ -531,38 +534,53 @@ __global__ __launch_bounds__(64) void cidx_fill(int32_t nbands, const Band* __re
   if (I >= nbands) return;
   const int lane = threadIdx.x;
   const Band& bI = bands[I];
-  const int p = lane;
+  // four tiles per pass: lane group `slot` (16 lanes) takes tile tb + slot,
+  // lane p of the group row atom p of band I
+  const int p = lane & 15, slot = lane >> 4;
   const bool act = p < bI.natoms;
   const int m = act ? bI.rs[p + 1] - bI.rs[p] : 0;
-  int64_t b = act ? row_p[bI.first + p] : 0;
+  int64_t b = act ? row_p[bI.first + p] : 0;  // next C block of row p (same in the 4 groups)
   unsigned long long ne = 0;
-  for (int64_t t = tile_base[I]; t < tile_base[I + 1]; ++t) {
-    const TileDesc* td = tiles + t;
+  const int64_t t1 = tile_base[I + 1];
+  for (int64_t tb = tile_base[I]; tb < t1; tb += 4) {
+    const int64_t t = tb + slot;
+    const bool tv = t < t1;
+    const TileDesc* td = tiles + (tv ? t : tb);
     const Band& bJ = bands[td->J];
-    const uint32_t pm = act ? td->pattern[p] : 0u;
+    const uint32_t pm = (act && tv) ? td->pattern[p] : 0u;
+    const uint32_t km = (act && tv) ? kept[t * kBandAtoms + p] : 0u;
     int64_t rowsz = 0;
     for (int q = 0; q < bJ.natoms; ++q)
       if ((pm >> q) & 1u) rowsz += (int64_t)m * (bJ.rs[q + 1] - bJ.rs[q]);
-    // exclusive prefix over the 16 row positions
+    // exclusive prefix over the 16 row positions of this tile
     int64_t x = rowsz;
     for (int o = 1; o < 16; o <<= 1) {
-      const int64_t y = __shfl_up(x, o, 64);
+      const int64_t y = __shfl_up(x, o, 16);
       if (p >= o) x += y;
     }
     int64_t o_run = td->out_base + x - rowsz;
-    const uint32_t km = act ? kept[t * kBandAtoms + p] : 0u;
+    // kept blocks of row p in the earlier tiles of this pass
+    const int nk = __builtin_popcount(pm & km);
+    int kx = nk;
+    for (int o = 16; o < 64; o <<= 1) {
+      const int y = __shfl_up(kx, o, 64);
+      if (lane >= o) kx += y;
+    }
+    const int total = __shfl(kx, 48 + p, 64);
+    int64_t bb = b + (kx - nk);
     for (int q = 0; q < bJ.natoms; ++q) {
       if (!((pm >> q) & 1u)) continue;
       const int64_t sz = (int64_t)m * (bJ.rs[q + 1] - bJ.rs[q]);
       if ((km >> q) & 1u) {
-        col[b] = bJ.first + q;
-        off[b] = o_run;
-        rows[b] = bI.first + p;
-        ++b;
+        col[bb] = bJ.first + q;
+        off[bb] = o_run;
+        rows[bb] = bI.first + p;
+        ++bb;
         ne += (unsigned long long)sz;
       }
       o_run += sz;
     }
+    b += total;
   }
   ne = wave_sum_i64((int64_t)ne);
   if (lane == 0) atomicAdd(nelems, ne);

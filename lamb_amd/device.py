@@ -202,6 +202,22 @@ class DeviceMatrix:
             self.h = None
 
 
+def partition_atoms(ctx: Context, coords, box, dims):
+    """(subset_map int32, order_map int16) of wrapped FP32 coordinates over
+    dims cubes, computed on the device (lamb_hip_partition_atoms)."""
+    xyz = np.ascontiguousarray(coords, dtype=np.float32).reshape(-1, 3)
+    h = np.ascontiguousarray(np.broadcast_to(np.asarray(box, np.float32), (3,)))
+    d = np.ascontiguousarray(dims, dtype=np.int32)
+    n = len(xyz)
+    subset = np.zeros(n, np.int32)
+    order = np.zeros(n, np.int16)
+    rc = ctx.lib.lamb_hip_partition_atoms(ctx.h, n, N.ptr(xyz, C.c_float), N.ptr(h, C.c_float),
+                                          N.ptr(d, C.c_int32), N.ptr(subset, C.c_int32),
+                                          N.ptr(order, C.c_int16))
+    ctx.check(rc, "partition_atoms")
+    return subset, order
+
+
 def tuning_info(ctx: Context):
     """(FP64 kernel variant of the last multiply, workload classes tuned)."""
     v, n = C.c_int32(), C.c_int32()

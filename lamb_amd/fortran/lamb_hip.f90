@@ -50,7 +50,7 @@ module lamb_hip
   public :: lamb_hip_matrix_create, lamb_hip_matrix_destroy, lamb_hip_matrix_copy
   public :: lamb_hip_sector_upload, lamb_hip_distribute, lamb_hip_matrix_commit
   public :: lamb_hip_sector_query, lamb_hip_sector_download, lamb_hip_matrix_info
-  public :: lamb_hip_multiply, lamb_hip_trace, lamb_hip_dbcsr_load
+  public :: lamb_hip_multiply, lamb_hip_trace, lamb_hip_dbcsr_load, lamb_hip_partition_atoms
   public :: lamb_multiply_hip, internal_distribute_hip, lamb_load_dbcsr_hip
 
   interface
@@ -193,6 +193,18 @@ module lamb_hip
       import :: c_int32_t, c_double, c_ptr
       type(c_ptr), value :: a
       real(c_double), intent(out) :: trace
+      integer(c_int32_t) :: st
+    end function
+
+    function lamb_hip_partition_atoms(ctx, n_atoms, coords, box, dims, subset_map, order_map) &
+        bind(C, name="lamb_hip_partition_atoms") result(st)
+      import :: c_int16_t, c_int32_t, c_int64_t, c_float, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: n_atoms
+      real(c_float), intent(in) :: coords(3, *), box(3)
+      integer(c_int32_t), intent(in) :: dims(3)
+      integer(c_int32_t), intent(out) :: subset_map(*)
+      integer(c_int16_t), intent(out) :: order_map(*)
       integer(c_int32_t) :: st
     end function
 
