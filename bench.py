@@ -43,7 +43,10 @@ def parse():
                         "config5: gradient of tr(P_k H) through the adjoint kernels")
     p.add_argument("--systems", type=int, default=64, help="config4: systems in the batch")
     p.add_argument("--lo", type=int, default=96, help="config4: smallest system (atoms)")
-    p.add_argument("--hi", type=int, default=3072, help="config4: largest system (atoms)")
+    p.add_argument("--hi", type=int, default=1024,
+                   help="config4: largest system (atoms). Iterates fill in to dense after one "
+                        "McWeeny step; with systems up to 3,072 atoms a dense iterate of the "
+                        "64-system batch is 78 GB and a step does not fit one GPU")
     p.add_argument("--chain-steps", type=int, default=0,
                    help="McWeeny steps (default 20 for config4, 10 for config5)")
     p.add_argument("--atoms", type=int, default=3072, help="config5: atoms of the water box")
