@@ -108,7 +108,8 @@ hipError_t launch_block_nsq(hipStream_t s, int64_t nblk, const int32_t* d_rows,
                             const int32_t* d_cols, const int32_t* d_blk, const int64_t* d_off,
                             const double* d_data, double* d_nsq);
 // FP64 numeric kernel variants: per-wave 32 x 32 quadrants, or whole-tile
-// waves splitting the K steps
+// waves splitting the K steps. whole: the product has no partial entry (the
+// K-split kernel then runs without per-lane masks)
 constexpr int kF64Quadrant = 0;
 constexpr int kF64Ksplit = 1;
 hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* tiles,
@@ -117,7 +118,7 @@ hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* ti
                               uint32_t b_zero, double alpha, double beta, double eps2,
                               const int64_t* old_row_p, const int32_t* old_col,
                               const int64_t* old_off, const double* old_data, double* c_data,
-                              uint16_t* kept, int variant);
+                              uint16_t* kept, int variant, bool whole);
 // BF16-split numeric phase (bf16split.hip). ex: int32 per matrix row (A,
 // row_side) or column (B); slice packs: slices x (n_chunks + 1) x 64 x 16 B.
 hipError_t launch_slice_exponents(hipStream_t s, int64_t nblk, const int32_t* rows,

@@ -368,6 +368,9 @@ __device__ __forceinline__ uint32_t tile_blocks(uint32_t g) {
   return bm;
 }
 
+// kWhole: every entry of the product covers its strips' atoms (no
+// kEntryPartial), so lanes load whole panel columns without masks.
+template <bool kWhole>
 __global__ __launch_bounds__(256, 2) void numeric_f64_ksplit_kernel(
     const TileDesc* __restrict__ tiles, const int32_t* __restrict__ perm,
     const Entry* __restrict__ entries, const Band* __restrict__ bands,
@@ -442,16 +445,27 @@ __global__ __launch_bounds__(256, 2) void numeric_f64_ksplit_kernel(
       blkmask[S] = bm;                                                             \
       const int Q = 4 * st + kc;                                                   \
       const uint2 ab = col_ab[Q];                                                  \
-      const uint32_t cm = col_m[Q];                                                \
-      _Pragma("unroll") for (int h = 0; h < 4; ++h) {                              \
-        const uint32_t need = (bm & (0xFu << (4 * h))) ? rbit[h] : 0u;             \
-        const uint32_t c = (cm & need) ? ab.x : a_zero;                            \
-        ra[S][h] = a_lane[(int64_t)c * 64 + 16 * h];                               \
-      }                                                                            \
-      _Pragma("unroll") for (int q = 0; q < 4; ++q) {                              \
-        const uint32_t need = (bm & (0x1111u << q)) ? cbit[q] : 0u;                \
-        const uint32_t c = (cm & need) ? ab.y : b_zero;                            \
-        rb[S][q] = b_lane[(int64_t)c * 64 + 16 * q];                               \
+      if constexpr (kWhole) {                                                      \
+        /* without a step of this wave the loads read step 0, feeding no MFMA */   \
+        const double* pa = a_lane + (int64_t)ab.x * 64;                            \
+        const double* pb = b_lane + (int64_t)ab.y * 64;                            \
+        _Pragma("unroll") for (int h = 0; h < 4; ++h) ra[S][h] = pa[16 * h];       \
+        _Pragma("unroll") for (int q = 0; q < 4; ++q) rb[S][q] = pb[16 * q];       \
+        /* keeps the scheduler from hoisting these loads over the MFMAs, */        \
+        /* which spilled 100+ VGPRs */                                             \
+        __builtin_amdgcn_sched_barrier(0);                                         \
+      } else {                                                                     \
+        const uint32_t cm = col_m[Q];                                              \
+        _Pragma("unroll") for (int h = 0; h < 4; ++h) {                            \
+          const uint32_t need = (bm & (0xFu << (4 * h))) ? rbit[h] : 0u;           \
+          const uint32_t c = (cm & need) ? ab.x : a_zero;                          \
+          ra[S][h] = a_lane[(int64_t)c * 64 + 16 * h];                             \
+        }                                                                          \
+        _Pragma("unroll") for (int q = 0; q < 4; ++q) {                            \
+          const uint32_t need = (bm & (0x1111u << q)) ? cbit[q] : 0u;              \
+          const uint32_t c = (cm & need) ? ab.y : b_zero;                          \
+          rb[S][q] = b_lane[(int64_t)c * 64 + 16 * q];                             \
+        }                                                                          \
       }                                                                            \
     }
 #define KS_MFMA(S)                                                                 \
@@ -511,12 +525,16 @@ hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* ti
                               uint32_t b_zero, double alpha, double beta, double eps2,
                               const int64_t* old_row_p, const int32_t* old_col,
                               const int64_t* old_off, const double* old_data, double* c_data,
-                              uint16_t* kept, int variant) {
+                              uint16_t* kept, int variant, bool whole) {
   if (n_tiles == 0) return hipSuccess;
-  if (variant == kF64Ksplit)
-    hipLaunchKernelGGL(numeric_f64_ksplit_kernel, dim3((unsigned)n_tiles), dim3(256), 0, s, tiles,
-                       perm, entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta, eps2,
-                       old_row_p, old_col, old_off, old_data, c_data, kept);
+  if (variant == kF64Ksplit && whole)
+    hipLaunchKernelGGL(numeric_f64_ksplit_kernel<true>, dim3((unsigned)n_tiles), dim3(256), 0, s,
+                       tiles, perm, entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta,
+                       eps2, old_row_p, old_col, old_off, old_data, c_data, kept);
+  else if (variant == kF64Ksplit)
+    hipLaunchKernelGGL(numeric_f64_ksplit_kernel<false>, dim3((unsigned)n_tiles), dim3(256), 0, s,
+                       tiles, perm, entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta,
+                       eps2, old_row_p, old_col, old_off, old_data, c_data, kept);
   else
     hipLaunchKernelGGL(numeric_f64_kernel, dim3((unsigned)n_tiles), dim3(256), 0, s, tiles, perm,
                        entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta, eps2,

@@ -286,20 +286,23 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
                               use_old ? MC.row_p : nullptr, use_old ? MC.col : nullptr,
                               use_old ? MC.off : nullptr,
                               use_old ? (const double*)C->d_data.p : nullptr,
-                              (double*)newdata.p, (uint16_t*)ctx->d_kept.p, variant);
+                              (double*)newdata.p, (uint16_t*)ctx->d_kept.p, variant,
+                              S.r.n_partial == 0);
   };
   int variant = 0;
   if (!split) {
     // Autotuner: the FP64 variants trade occupancy (quadrant) against work
     // per K step (ksplit); which wins depends on how dense the tiles are. A
-    // workload class is (form, largest block, flops per tile, C blocks per
-    // tile) in half-octave buckets; the first multiply of a class times every
-    // variant on its own tiles (the launches only rewrite this call's C pool)
-    // and the winner is kept for the context's lifetime.
+    // workload class is (form and whether every entry is whole, largest
+    // block, flops per tile, C blocks per tile) in half-octave buckets; the
+    // first multiply of a class times every variant on its own tiles (the
+    // launches only rewrite this call's C pool) and the winner is kept for
+    // the context's lifetime.
     variant = f64_forced();
     if (variant < 0) {
       const double nt = std::max<double>(1.0, (double)n_tiles);
-      const std::array<int64_t, 4> key = {flags, d->max_blk, bucket(S.r.flops / nt),
+      const int64_t form = (int64_t)flags | ((int64_t)(S.r.n_partial == 0) << 32);
+      const std::array<int64_t, 4> key = {form, d->max_blk, bucket(S.r.flops / nt),
                                            bucket((double)S.r.n_struct / nt)};
       auto it = ctx->tune.find(key);
       if (it != ctx->tune.end()) {

@@ -57,6 +57,7 @@ struct SymScratch {
 struct SymResult {
   int64_t n_tiles = 0, n_entries = 0, c_size = 0, n_struct = 0;
   int64_t kept = 0, dropped = 0;
+  int64_t n_partial = 0;  // entries flagged kEntryPartial
   double flops = 0.0;
   DevBuf tiles, entries, tile_base, perm;
   void release() {

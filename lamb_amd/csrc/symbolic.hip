@@ -274,7 +274,7 @@ __global__ __launch_bounds__(64) void sym_count(
   __shared__ int16_t rsI[kBandAtoms + 1];
   if (lane <= kBandAtoms) rsI[lane] = bands[I].rs[min(lane, bands[I].natoms)];
   __syncthreads();
-  unsigned long long kept = 0, dropped = 0;
+  unsigned long long kept = 0, dropped = 0, partial = 0;
   double flops = 0.0;
   for (int64_t sa = lo; sa < hi; ++sa) {
     const Strip* SA = as + sa;
@@ -290,6 +290,9 @@ __global__ __launch_bounds__(64) void sym_count(
         if (!mask_entry(mpat, nbands, I, J, rm, cm, pairs)) return;
         // a zero-size K atom adds to the C pattern but holds no entry
         if (kd > 0) ++n;
+        // entries whose atoms the prefilter or the mask cut (sym_fill flags
+        // them with kEntryPartial)
+        if (kd > 0 && (rm != SA->mask || cm != SB->mask)) ++partial;
         kept += (unsigned long long)__builtin_popcount(rm) * __builtin_popcount(cm);
         // rows of cm in band J
         int cr = 0;
@@ -324,12 +327,14 @@ __global__ __launch_bounds__(64) void sym_count(
   for (int o = 32; o > 0; o >>= 1) {
     kept += __shfl_xor(kept, o, 64);
     dropped += __shfl_xor(dropped, o, 64);
+    partial += __shfl_xor(partial, o, 64);
     flops += __shfl_xor(flops, o, 64);
   }
   if (lane == 0) {
     atomicAdd(&stats[0], kept);
     atomicAdd(&stats[1], dropped);
     atomicAdd(&stats[2], (unsigned long long)flops);
+    if (partial) atomicAdd(&stats[3], partial);
   }
 }
 
@@ -621,13 +626,13 @@ __global__ void tile_perm(int64_t ntiles, int rb, int32_t nbands, const TileDesc
 }
 
 // out[0..3] = the four band-total sums (after their scans, at [v*(nb+2)+nb]),
-// out[4..6] = kept, dropped, flops statistics
+// out[4..7] = kept, dropped, flops and partial-entry statistics
 __global__ void gather_totals(const int64_t* __restrict__ bts, int32_t nb,
                               const int64_t* __restrict__ stats, int64_t* __restrict__ out) {
   LAMB_GUARD;
   const int t = threadIdx.x;
   if (t < 4) out[t] = bts[(int64_t)t * (nb + 2) + nb];
-  else if (t < 7) out[t] = stats[t - 4];
+  else if (t < 8) out[t] = stats[t - 4];
 }
 
 // ---------------------------------------------------------------------------
@@ -751,13 +756,13 @@ hipError_t build_tiles(lamb_hip_ctx* ctx, SymScratch& W, const DistView& D, cons
   CK(W.scan.ensure(sizeof(int64_t) * (size_t)scan_scratch_elems(nb + 1)));
   for (int v = 0; v < 4; ++v)
     CK(exclusive_scan(s, bt + v * (nb + 1), bts + v * (nb + 2), nb, (int64_t*)W.scan.p));
-  // the four totals and three statistics, gathered for one read
+  // the four totals and four statistics, gathered for one read
   int64_t* gathered = (int64_t*)W.stats.p + 8;
   hipLaunchKernelGGL(gather_totals, dim3(1), dim3(64), 0, s, bts, nb,
                      (const int64_t*)W.stats.p, gathered);
   CK(hipGetLastError());
-  int64_t tot[7];
-  CK(read_sizes(ctx, gathered, tot, 7));
+  int64_t tot[8];
+  CK(read_sizes(ctx, gathered, tot, 8));
   r.n_tiles = tot[0];
   r.n_entries = tot[1];
   r.c_size = tot[2];
@@ -765,6 +770,7 @@ hipError_t build_tiles(lamb_hip_ctx* ctx, SymScratch& W, const DistView& D, cons
   r.kept = tot[4];
   r.dropped = tot[5];
   r.flops = (double)(unsigned long long)tot[6];
+  r.n_partial = tot[7];
   CK(r.tiles.ensure(sizeof(TileDesc) * (size_t)std::max<int64_t>(r.n_tiles, 1)));
   CK(r.entries.ensure(sizeof(Entry) * (size_t)std::max<int64_t>(r.n_entries, 1)));
   CK(r.tile_base.ensure(sizeof(int64_t) * (size_t)(nb + 1)));

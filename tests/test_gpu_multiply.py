@@ -194,3 +194,18 @@ def test_autotuner_is_deterministic_and_variants_agree(monkeypatch):
             assert same and err <= TOL, (name, err)
     finally:
         fresh.close()
+
+
+@pytest.mark.parametrize("variant", ["quadrant", "ksplit"])
+def test_forced_variants_whole_and_partial_products(ctx, monkeypatch, variant):
+    """Each FP64 variant on a product whose entries are all whole (unmasked
+    loads) and on one where the prefilter cuts entries (per-lane masks)."""
+    monkeypatch.setenv("LAMB_HIP_F64_KERNEL", variant)
+    atoms = S.water_system(150, seed=111)
+    a = S.decay_matrix(atoms, 8.0, 0.4, 112)
+    b = S.decay_matrix(atoms, 8.0, 0.4, 113)
+    check(ctx, atoms, a, b, eps=1e-7)
+    atoms, a = S.small_system(60, 31, cutoff=9.0, decay=1.5, sizes=(1, 5, 13))
+    b = S.decay_matrix(atoms, 9.0, 1.3, 32)
+    st, rst = check(ctx, atoms, a, b, eps=2e-3)
+    assert rst["n_triples_filtered"] > 0
