@@ -261,9 +261,9 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
     return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_OVERFLOW_ERROR,
                      "multiply: symbolic sizes exceed 32-bit indices");
   {
-    // tiles run in blocks of rb row bands (tuning knob, default 16)
+    // tiles run in blocks of rb row bands (tuning knob, default 8)
     const char* env = getenv("LAMB_HIP_TILE_ROWBLOCK");
-    const int rb = env ? atoi(env) : 16;
+    const int rb = env ? atoi(env) : 8;
     HIPCHK(ctx, build_tile_order(ctx, S.W, D, S.r, std::max(1, std::min(rb, 32))), "tile order");
   }
   HIPCHK(ctx, record_event(ctx, 3), "event");

@@ -31,18 +31,20 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Tile of this workgroup and its band geometry in LDS. Workgroups are dealt
-// round-robin over the 8 XCDs; the remap lets each XCD walk one contiguous
-// range of the tile order perm. Ends with the LDS visible to all threads.
+// Tile of this workgroup and its band geometry in LDS. Workgroup b runs
+// perm[b]. The hardware deals consecutive workgroups round-robin over the 8
+// XCDs, so the tiles running at any moment are a window of the tile order
+// spread over all XCDs: the A and B strips of that window are read by all of
+// them from the shared last-level cache. (Giving each XCD its own contiguous
+// range of the order, so that each L2 sees a compact range, measured 20.1 ms
+// on the 6,144-atom box against 18.5 ms.) Ends with the LDS visible to all
+// threads.
 __device__ __forceinline__ TileInfo tile_setup(const TileDesc* __restrict__ tiles,
                                                const int32_t* __restrict__ perm,
                                                const Band* __restrict__ bands, TileSmem& sm) {
   const int tid = threadIdx.x;
-  const int nb = gridDim.x;
-  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-  const int per = nb >> 3, rem = nb & 7;
   TileInfo t;
-  t.tile = perm[(xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per) + slot];
+  t.tile = perm[blockIdx.x];
   t.tp = tiles + t.tile;
   t.I = t.tp->I;
   t.J = t.tp->J;
