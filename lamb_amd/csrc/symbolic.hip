@@ -251,7 +251,13 @@ __global__ void mask_pattern(int64_t nblk, const int32_t* __restrict__ rows,
            (1u << q) << (16 * (p & 1)));
 }
 
-__global__ __launch_bounds__(64) void sym_count(
+// kLds = false: one wave per (band I, segment g), pattern words ORed into
+// global memory. kLds = true: one workgroup of nseg waves per band I (wave g
+// = segment g); the band's nbands x 8 pattern words are ORed in LDS (dynamic
+// shared memory, 32 * nbands bytes) and stored to global memory at the end,
+// which replaces most of the global atomics.
+template <bool kLds>
+__global__ __launch_bounds__(kLds ? 512 : 64) void sym_count(
     int32_t nbands, int nseg, const Band* __restrict__ bands, const int32_t* __restrict__ blk,
     const Strip* __restrict__ as, const int64_t* __restrict__ a_ptr,
     const Strip* __restrict__ bs, const int64_t* __restrict__ b_ptr, double eps2,
@@ -260,20 +266,29 @@ __global__ __launch_bounds__(64) void sym_count(
     int32_t* __restrict__ cnt, int32_t* __restrict__ ksum, uint32_t* __restrict__ pat,
     unsigned long long* __restrict__ stats, const uint32_t* __restrict__ mpat) {
   LAMB_GUARD;
-  const int64_t w = blockIdx.x;
-  const int32_t I = (int32_t)(w / nseg);
-  const int g = (int)(w % nseg);
+  extern __shared__ uint32_t spat[];
+  const int32_t I = kLds ? (int32_t)blockIdx.x : (int32_t)(blockIdx.x / nseg);
+  const int g = kLds ? (int)(threadIdx.x >> 6) : (int)(blockIdx.x % nseg);
   if (I >= nbands) return;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   const int64_t sa0 = a_ptr[I], sa1 = a_ptr[I + 1];
   const int64_t len = sa1 - sa0;
   const int64_t lo = sa0 + len * g / nseg, hi = sa0 + len * (g + 1) / nseg;
   int32_t* c = cnt + ((int64_t)I * nseg + g) * nbands;
   int32_t* ks = ksum + ((int64_t)I * nseg + g) * nbands;
   uint32_t* pt = pat + (int64_t)I * nbands * (kBandAtoms / 2);
+  const int64_t nwords = (int64_t)nbands * (kBandAtoms / 2);
   __shared__ int16_t rsI[kBandAtoms + 1];
-  if (lane <= kBandAtoms) rsI[lane] = bands[I].rs[min(lane, bands[I].natoms)];
+  if (threadIdx.x <= kBandAtoms) rsI[threadIdx.x] = bands[I].rs[min((int)threadIdx.x, bands[I].natoms)];
+  if (kLds)
+    for (int64_t x = threadIdx.x; x < nwords; x += blockDim.x) spat[x] = 0u;
   __syncthreads();
+  auto or_word = [&](int64_t x, uint32_t v) {
+    if constexpr (kLds)
+      atomicOr(&spat[x], v);
+    else
+      atomicOr(&pt[x], v);
+  };
   unsigned long long kept = 0, dropped = 0, partial = 0;
   double flops = 0.0;
   for (int64_t sa = lo; sa < hi; ++sa) {
@@ -302,7 +317,7 @@ __global__ __launch_bounds__(64) void sym_count(
         // one atomic per pattern word (two row atoms)
         for (int wd = 0; wd < kBandAtoms / 2; ++wd) {
           const uint32_t v = pairs[2 * wd] | (pairs[2 * wd + 1] << 16);
-          if (v) atomicOr(&pt[(int64_t)J * (kBandAtoms / 2) + wd], v);
+          if (v) or_word((int64_t)J * (kBandAtoms / 2) + wd, v);
         }
       });
       if (n) {
@@ -318,8 +333,7 @@ __global__ __launch_bounds__(64) void sym_count(
       for (int64_t b = old_row_p[i] + lane; b < old_row_p[i + 1]; b += 64) {
         const int32_t j = old_col[b];
         const int32_t J = band_of[j];
-        atomicOr(&pt[(int64_t)J * (kBandAtoms / 2) + (p >> 1)],
-                 (1u << pos_in_band[j]) << (16 * (p & 1)));
+        or_word((int64_t)J * (kBandAtoms / 2) + (p >> 1), (1u << pos_in_band[j]) << (16 * (p & 1)));
       }
     }
   }
@@ -335,6 +349,10 @@ __global__ __launch_bounds__(64) void sym_count(
     atomicAdd(&stats[1], dropped);
     atomicAdd(&stats[2], (unsigned long long)flops);
     if (partial) atomicAdd(&stats[3], partial);
+  }
+  if (kLds) {
+    __syncthreads();
+    for (int64_t x = threadIdx.x; x < nwords; x += blockDim.x) pt[x] = spat[x];
   }
 }
 
@@ -733,15 +751,28 @@ hipError_t build_tiles(lamb_hip_ctx* ctx, SymScratch& W, const DistView& D, cons
   CK(W.stats.ensure(sizeof(unsigned long long) * 16));
   CK(fill32(s, (int32_t*)W.cnt.p, dense, 0));
   CK(fill32(s, (int32_t*)W.ksum.p, dense, 0));
-  CK(fill32(s, (int32_t*)W.pat.p, (int64_t)nb * nb * (kBandAtoms / 2), 0));
+  // pattern words in LDS when a band's row of them fits (32 bytes per band)
+  const size_t lds_pat = sizeof(uint32_t) * (size_t)nb * (kBandAtoms / 2);
+  const bool pat_lds = lds_pat <= 96 * 1024;
+  if (!pat_lds) CK(fill32(s, (int32_t*)W.pat.p, (int64_t)nb * nb * (kBandAtoms / 2), 0));
   CK(fill32(s, (int32_t*)W.stats.p, 8, 0));
   if (nb)
-    hipLaunchKernelGGL(sym_count, dim3((unsigned)(nb * nseg)), dim3(64), 0, s, nb, nseg, D.bands,
+    {
+      if (pat_lds)
+        hipLaunchKernelGGL(sym_count<true>, dim3((unsigned)nb), dim3(64 * nseg), lds_pat, s, nb, nseg, D.bands,
                        D.blk, (const Strip*)A.strips.p, (const int64_t*)A.ptr.p,
                        (const Strip*)B.strips.p, (const int64_t*)B.ptr.p, eps2,
                        Cold ? Cold->row_p : nullptr, Cold ? Cold->col : nullptr, D.band_of,
                        D.pos_in_band, (int32_t*)W.cnt.p, (int32_t*)W.ksum.p,
                        (uint32_t*)W.pat.p, (unsigned long long*)W.stats.p, mp);
+      else
+        hipLaunchKernelGGL(sym_count<false>, dim3((unsigned)(nb * nseg)), dim3(64), 0, s, nb, nseg, D.bands,
+                       D.blk, (const Strip*)A.strips.p, (const int64_t*)A.ptr.p,
+                       (const Strip*)B.strips.p, (const int64_t*)B.ptr.p, eps2,
+                       Cold ? Cold->row_p : nullptr, Cold ? Cold->col : nullptr, D.band_of,
+                       D.pos_in_band, (int32_t*)W.cnt.p, (int32_t*)W.ksum.p,
+                       (uint32_t*)W.pat.p, (unsigned long long*)W.stats.p, mp);
+    }
   CK(hipGetLastError());
   // per-band totals and their scans
   CK(W.band4.ensure(sizeof(int64_t) * 4 * (size_t)(nb + 1)));
