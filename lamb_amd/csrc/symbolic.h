@@ -46,10 +46,10 @@ struct StripSet {
 
 struct SymScratch {
   DevBuf merged, flag, excl, scan, gslot;
-  DevBuf cnt, ksum, pat, stats, band4, band4s, cells, cellcnt, cellbase, mpat;
+  DevBuf cnt, ksum, pat, stats, band4, band4s, cells, cellcnt, cellbase, mpat, cidx;
   void release() {
     for (DevBuf* b : {&merged, &flag, &excl, &scan, &gslot, &cnt, &ksum, &pat, &stats, &band4,
-                      &band4s, &cells, &cellcnt, &cellbase, &mpat})
+                      &band4s, &cells, &cellcnt, &cellbase, &mpat, &cidx})
       b->release();
   }
 };

@@ -251,11 +251,12 @@ __global__ void mask_pattern(int64_t nblk, const int32_t* __restrict__ rows,
            (1u << q) << (16 * (p & 1)));
 }
 
-// kLds = false: one wave per (band I, segment g), pattern words ORed into
-// global memory. kLds = true: one workgroup of nseg waves per band I (wave g
-// = segment g); the band's nbands x 8 pattern words are ORed in LDS (dynamic
-// shared memory, 32 * nbands bytes) and stored to global memory at the end,
-// which replaces most of the global atomics.
+// kLds = false: one wave per (band I, segment g), pattern words and
+// counters updated in global memory. kLds = true: one workgroup of nseg waves
+// per band I (wave g = segment g); the band's nbands x 8 pattern words and
+// nseg x nbands entry / K-column counters live in LDS (dynamic shared memory,
+// sym_count_lds(nbands, nseg) bytes) and are stored to global memory at the
+// end, in place of global atomics.
 template <bool kLds>
 __global__ __launch_bounds__(kLds ? 512 : 64) void sym_count(
     int32_t nbands, int nseg, const Band* __restrict__ bands, const int32_t* __restrict__ blk,
@@ -274,14 +275,17 @@ __global__ __launch_bounds__(kLds ? 512 : 64) void sym_count(
   const int64_t sa0 = a_ptr[I], sa1 = a_ptr[I + 1];
   const int64_t len = sa1 - sa0;
   const int64_t lo = sa0 + len * g / nseg, hi = sa0 + len * (g + 1) / nseg;
-  int32_t* c = cnt + ((int64_t)I * nseg + g) * nbands;
-  int32_t* ks = ksum + ((int64_t)I * nseg + g) * nbands;
-  uint32_t* pt = pat + (int64_t)I * nbands * (kBandAtoms / 2);
   const int64_t nwords = (int64_t)nbands * (kBandAtoms / 2);
+  int32_t* scnt = reinterpret_cast<int32_t*>(spat + nwords);  // [nseg][nbands]
+  int32_t* sks = scnt + (int64_t)nseg * nbands;                // [nseg][nbands]
+  int32_t* c = kLds ? scnt + (int64_t)g * nbands : cnt + ((int64_t)I * nseg + g) * nbands;
+  int32_t* ks = kLds ? sks + (int64_t)g * nbands : ksum + ((int64_t)I * nseg + g) * nbands;
+  uint32_t* pt = pat + (int64_t)I * nbands * (kBandAtoms / 2);
   __shared__ int16_t rsI[kBandAtoms + 1];
   if (threadIdx.x <= kBandAtoms) rsI[threadIdx.x] = bands[I].rs[min((int)threadIdx.x, bands[I].natoms)];
   if (kLds)
-    for (int64_t x = threadIdx.x; x < nwords; x += blockDim.x) spat[x] = 0u;
+    for (int64_t x = threadIdx.x; x < nwords + 2 * (int64_t)nseg * nbands; x += blockDim.x)
+      spat[x] = 0u;
   __syncthreads();
   auto or_word = [&](int64_t x, uint32_t v) {
     if constexpr (kLds)
@@ -353,8 +357,24 @@ __global__ __launch_bounds__(kLds ? 512 : 64) void sym_count(
   if (kLds) {
     __syncthreads();
     for (int64_t x = threadIdx.x; x < nwords; x += blockDim.x) pt[x] = spat[x];
+    int32_t* gc = cnt + (int64_t)I * nseg * nbands;
+    int32_t* gk = ksum + (int64_t)I * nseg * nbands;
+    for (int64_t x = threadIdx.x; x < (int64_t)nseg * nbands; x += blockDim.x) {
+      gc[x] = scnt[x];
+      gk[x] = sks[x];
+    }
   }
 }
+
+// dynamic LDS of sym_count<true> / sym_fill<true>
+static size_t sym_count_lds(int32_t nbands, int nseg) {
+  return sizeof(uint32_t) * (size_t)nbands * (kBandAtoms / 2) +
+         2 * sizeof(int32_t) * (size_t)nseg * (size_t)nbands;
+}
+static size_t sym_fill_lds(int32_t nbands, int nseg) {
+  return 2 * sizeof(int32_t) * (size_t)nseg * (size_t)nbands;
+}
+constexpr size_t kSymLdsMax = 96 * 1024;
 
 // Per band: tiles present, their entry counts and C sizes (lanes over J).
 __global__ __launch_bounds__(64) void sym_band_totals(
@@ -470,23 +490,36 @@ __global__ __launch_bounds__(64) void sym_tiles(
   }
 }
 
-__global__ __launch_bounds__(64) void sym_fill(
+// kLds = false: one wave per (band I, segment g) with its cursors in global
+// memory. kLds = true: one workgroup of nseg waves per band I, the band's
+// cursors copied to LDS (sym_fill_lds(nbands, nseg) bytes of dynamic shared
+// memory) so that reserving entry slots takes LDS atomics.
+template <bool kLds>
+__global__ __launch_bounds__(kLds ? 512 : 64) void sym_fill(
     int32_t nbands, int nseg, const int32_t* __restrict__ blk, const Strip* __restrict__ as,
     const int64_t* __restrict__ a_ptr, const Strip* __restrict__ bs,
     const int64_t* __restrict__ b_ptr, const int64_t* __restrict__ a_pk,
     const int64_t* __restrict__ b_pk, double eps2, int32_t* __restrict__ cursor,
     int32_t* __restrict__ kcur, Entry* __restrict__ entries, const uint32_t* __restrict__ mpat) {
   LAMB_GUARD;
-  const int64_t w = blockIdx.x;
-  const int32_t I = (int32_t)(w / nseg);
-  const int g = (int)(w % nseg);
+  extern __shared__ int32_t scur[];
+  const int32_t I = kLds ? (int32_t)blockIdx.x : (int32_t)(blockIdx.x / nseg);
+  const int g = kLds ? (int)(threadIdx.x >> 6) : (int)(blockIdx.x % nseg);
   if (I >= nbands) return;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   const int64_t sa0 = a_ptr[I], sa1 = a_ptr[I + 1];
   const int64_t len = sa1 - sa0;
   const int64_t lo = sa0 + len * g / nseg, hi = sa0 + len * (g + 1) / nseg;
-  int32_t* c = cursor + ((int64_t)I * nseg + g) * nbands;
-  int32_t* kc = kcur + ((int64_t)I * nseg + g) * nbands;
+  const int64_t nsb = (int64_t)nseg * nbands;
+  if (kLds) {
+    for (int64_t x = threadIdx.x; x < nsb; x += blockDim.x) {
+      scur[x] = cursor[(int64_t)I * nsb + x];
+      scur[nsb + x] = kcur[(int64_t)I * nsb + x];
+    }
+    __syncthreads();
+  }
+  int32_t* c = kLds ? scur + (int64_t)g * nbands : cursor + ((int64_t)I * nseg + g) * nbands;
+  int32_t* kc = kLds ? scur + nsb + (int64_t)g * nbands : kcur + ((int64_t)I * nseg + g) * nbands;
   for (int64_t sa = lo; sa < hi; ++sa) {
     const Strip* SA = as + sa;
     const int32_t k = SA->k;
@@ -525,35 +558,60 @@ __global__ __launch_bounds__(64) void sym_fill(
 }
 
 // ---------------------------------------------------------------------------
-// C index from the numeric kernel's kept masks. One wave per row band I,
-// lanes 0..15 = row positions; the band's tiles are visited in J order.
+// C index from the numeric kernel's kept masks. A band's tiles (J order)
+// are cut into kCidxSeg segments; one wave per (row band I, segment),
+// lanes 0..15 = row positions.
 // ---------------------------------------------------------------------------
+constexpr int kCidxSeg = 8;
+
+__device__ __forceinline__ void cidx_segment(const int64_t* __restrict__ tile_base, int32_t I,
+                                             int g, int64_t& t0, int64_t& t1) {
+  const int64_t a = tile_base[I], n = tile_base[I + 1] - a;
+  t0 = a + n * g / kCidxSeg;
+  t1 = a + n * (g + 1) / kCidxSeg;
+}
+
+// seg_cnt[(I * kCidxSeg + g) * 16 + p] = kept blocks of row p in segment g,
+// row_cnt[first + p] = their sum. One wave per band: lane (p, q) counts
+// segments q and q + 4 of row p.
 __global__ __launch_bounds__(64) void cidx_count(int32_t nbands, const Band* __restrict__ bands,
                                                  const int64_t* __restrict__ tile_base,
                                                  const uint16_t* __restrict__ kept,
+                                                 int32_t* __restrict__ seg_cnt,
                                                  int64_t* __restrict__ row_cnt) {
   LAMB_GUARD;
   const int32_t I = blockIdx.x;
   if (I >= nbands) return;
   const int lane = threadIdx.x;
+  const int p = lane & 15, q = lane >> 4;  // row p; segments q and q + 4
   const int nat = bands[I].natoms;
-  int64_t n = 0;
-  if (lane < nat)
-    for (int64_t t = tile_base[I]; t < tile_base[I + 1]; ++t)
-      n += __builtin_popcount(kept[t * kBandAtoms + lane]);
-  if (lane < nat) row_cnt[bands[I].first + lane] = n;
+  int32_t n[2] = {0, 0};
+  if (p < nat)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      int64_t t0, t1;
+      cidx_segment(tile_base, I, q + 4 * h, t0, t1);
+      for (int64_t t = t0; t < t1; ++t) n[h] += __builtin_popcount(kept[t * kBandAtoms + p]);
+      seg_cnt[((int64_t)I * kCidxSeg + q + 4 * h) * 16 + p] = n[h];
+    }
+  int64_t tot = (int64_t)n[0] + n[1];
+  tot += __shfl_xor(tot, 16, 64);
+  tot += __shfl_xor(tot, 32, 64);
+  if (lane < nat) row_cnt[bands[I].first + lane] = tot;
 }
 
 __global__ __launch_bounds__(64) void cidx_fill(int32_t nbands, const Band* __restrict__ bands,
                                                 const int64_t* __restrict__ tile_base,
                                                 const TileDesc* __restrict__ tiles,
                                                 const uint16_t* __restrict__ kept,
+                                                const int32_t* __restrict__ seg_cnt,
                                                 const int64_t* __restrict__ row_p,
                                                 int32_t* __restrict__ col, int64_t* __restrict__ off,
                                                 int32_t* __restrict__ rows,
                                                 unsigned long long* __restrict__ nelems) {
   LAMB_GUARD;
-  const int32_t I = blockIdx.x;
+  const int32_t I = (int32_t)(blockIdx.x / kCidxSeg);
+  const int g = (int)(blockIdx.x % kCidxSeg);
   if (I >= nbands) return;
   const int lane = threadIdx.x;
   const Band& bI = bands[I];
@@ -562,10 +620,16 @@ __global__ __launch_bounds__(64) void cidx_fill(int32_t nbands, const Band* __re
   const int p = lane & 15, slot = lane >> 4;
   const bool act = p < bI.natoms;
   const int m = act ? bI.rs[p + 1] - bI.rs[p] : 0;
-  int64_t b = act ? row_p[bI.first + p] : 0;  // next C block of row p (same in the 4 groups)
+  // next C block of row p (same in the 4 groups): after the earlier segments
+  int64_t b = 0;
+  if (act) {
+    b = row_p[bI.first + p];
+    for (int h = 0; h < g; ++h) b += seg_cnt[((int64_t)I * kCidxSeg + h) * 16 + p];
+  }
   unsigned long long ne = 0;
-  const int64_t t1 = tile_base[I + 1];
-  for (int64_t tb = tile_base[I]; tb < t1; tb += 4) {
+  int64_t t0, t1;
+  cidx_segment(tile_base, I, g, t0, t1);
+  for (int64_t tb = t0; tb < t1; tb += 4) {
     const int64_t t = tb + slot;
     const bool tv = t < t1;
     const TileDesc* td = tiles + (tv ? t : tb);
@@ -749,30 +813,31 @@ hipError_t build_tiles(lamb_hip_ctx* ctx, SymScratch& W, const DistView& D, cons
   CK(W.ksum.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(dense, 1)));
   CK(W.pat.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>((int64_t)nb * nb * (kBandAtoms / 2), 1)));
   CK(W.stats.ensure(sizeof(unsigned long long) * 16));
-  CK(fill32(s, (int32_t*)W.cnt.p, dense, 0));
-  CK(fill32(s, (int32_t*)W.ksum.p, dense, 0));
-  // pattern words in LDS when a band's row of them fits (32 bytes per band)
-  const size_t lds_pat = sizeof(uint32_t) * (size_t)nb * (kBandAtoms / 2);
-  const bool pat_lds = lds_pat <= 96 * 1024;
-  if (!pat_lds) CK(fill32(s, (int32_t*)W.pat.p, (int64_t)nb * nb * (kBandAtoms / 2), 0));
+  // a band's pattern words and counters in LDS when they fit
+  const size_t lds_cnt = sym_count_lds(nb, nseg);
+  const bool cnt_lds = lds_cnt <= kSymLdsMax;
+  if (!cnt_lds) {
+    CK(fill32(s, (int32_t*)W.cnt.p, dense, 0));
+    CK(fill32(s, (int32_t*)W.ksum.p, dense, 0));
+    CK(fill32(s, (int32_t*)W.pat.p, (int64_t)nb * nb * (kBandAtoms / 2), 0));
+  }
   CK(fill32(s, (int32_t*)W.stats.p, 8, 0));
-  if (nb)
-    {
-      if (pat_lds)
-        hipLaunchKernelGGL(sym_count<true>, dim3((unsigned)nb), dim3(64 * nseg), lds_pat, s, nb, nseg, D.bands,
+  if (nb) {
+    if (cnt_lds)
+      hipLaunchKernelGGL(sym_count<true>, dim3((unsigned)nb), dim3(64 * nseg), lds_cnt, s, nb, nseg, D.bands,
                        D.blk, (const Strip*)A.strips.p, (const int64_t*)A.ptr.p,
                        (const Strip*)B.strips.p, (const int64_t*)B.ptr.p, eps2,
                        Cold ? Cold->row_p : nullptr, Cold ? Cold->col : nullptr, D.band_of,
                        D.pos_in_band, (int32_t*)W.cnt.p, (int32_t*)W.ksum.p,
                        (uint32_t*)W.pat.p, (unsigned long long*)W.stats.p, mp);
-      else
-        hipLaunchKernelGGL(sym_count<false>, dim3((unsigned)(nb * nseg)), dim3(64), 0, s, nb, nseg, D.bands,
+    else
+      hipLaunchKernelGGL(sym_count<false>, dim3((unsigned)(nb * nseg)), dim3(64), 0, s, nb, nseg, D.bands,
                        D.blk, (const Strip*)A.strips.p, (const int64_t*)A.ptr.p,
                        (const Strip*)B.strips.p, (const int64_t*)B.ptr.p, eps2,
                        Cold ? Cold->row_p : nullptr, Cold ? Cold->col : nullptr, D.band_of,
                        D.pos_in_band, (int32_t*)W.cnt.p, (int32_t*)W.ksum.p,
                        (uint32_t*)W.pat.p, (unsigned long long*)W.stats.p, mp);
-    }
+  }
   CK(hipGetLastError());
   // per-band totals and their scans
   CK(W.band4.ensure(sizeof(int64_t) * 4 * (size_t)(nb + 1)));
@@ -811,12 +876,21 @@ hipError_t build_tiles(lamb_hip_ctx* ctx, SymScratch& W, const DistView& D, cons
                        (int32_t*)W.ksum.p, (const uint32_t*)W.pat.p, bts, bts + (nb + 2),
                        bts + 2 * (nb + 2), (TileDesc*)r.tiles.p);
   CK(hipGetLastError());
-  if (nb)
-    hipLaunchKernelGGL(sym_fill, dim3((unsigned)(nb * nseg)), dim3(64), 0, s, nb, nseg, D.blk,
-                       (const Strip*)A.strips.p, (const int64_t*)A.ptr.p,
-                       (const Strip*)B.strips.p, (const int64_t*)B.ptr.p,
-                       (const int64_t*)A.pk.p, (const int64_t*)B.pk.p, eps2, (int32_t*)W.cnt.p,
-                       (int32_t*)W.ksum.p, (Entry*)r.entries.p, mp);
+  if (nb) {
+    const size_t lds_fill = sym_fill_lds(nb, nseg);
+    if (lds_fill <= kSymLdsMax)
+      hipLaunchKernelGGL(sym_fill<true>, dim3((unsigned)nb), dim3(64 * nseg), lds_fill, s, nb,
+                         nseg, D.blk, (const Strip*)A.strips.p, (const int64_t*)A.ptr.p,
+                         (const Strip*)B.strips.p, (const int64_t*)B.ptr.p,
+                         (const int64_t*)A.pk.p, (const int64_t*)B.pk.p, eps2,
+                         (int32_t*)W.cnt.p, (int32_t*)W.ksum.p, (Entry*)r.entries.p, mp);
+    else
+      hipLaunchKernelGGL(sym_fill<false>, dim3((unsigned)(nb * nseg)), dim3(64), 0, s, nb, nseg,
+                         D.blk, (const Strip*)A.strips.p, (const int64_t*)A.ptr.p,
+                         (const Strip*)B.strips.p, (const int64_t*)B.ptr.p,
+                         (const int64_t*)A.pk.p, (const int64_t*)B.pk.p, eps2,
+                         (int32_t*)W.cnt.p, (int32_t*)W.ksum.p, (Entry*)r.entries.p, mp);
+  }
   return hipGetLastError();
 }
 
@@ -828,9 +902,11 @@ hipError_t build_c_index(lamb_hip_ctx* ctx, SymScratch& W, const DistView& D,
   CK(W.flag.ensure(sizeof(int64_t) * (size_t)(D.n + 1)));
   CK(W.scan.ensure(sizeof(int64_t) * (size_t)scan_scratch_elems(D.n + 1)));
   CK(W.stats.ensure(sizeof(unsigned long long) * 4));
+  CK(W.cidx.ensure(sizeof(int32_t) * (size_t)kCidxSeg * 16 * (size_t)std::max(nb, 1)));
   if (nb)
     hipLaunchKernelGGL(cidx_count, dim3(nb), dim3(64), 0, s, nb, D.bands,
-                       (const int64_t*)r.tile_base.p, kept, (int64_t*)W.flag.p);
+                       (const int64_t*)r.tile_base.p, kept, (int32_t*)W.cidx.p,
+                       (int64_t*)W.flag.p);
   CK(hipGetLastError());
   CK(exclusive_scan(s, (const int64_t*)W.flag.p, (int64_t*)c.row_p.p, D.n, (int64_t*)W.scan.p));
   int64_t nblk = 0;
@@ -841,10 +917,10 @@ hipError_t build_c_index(lamb_hip_ctx* ctx, SymScratch& W, const DistView& D,
   CK(c.rows.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(nblk, 1)));
   CK(fill32(s, (int32_t*)W.stats.p, 2, 0));
   if (nb)
-    hipLaunchKernelGGL(cidx_fill, dim3(nb), dim3(64), 0, s, nb, D.bands,
+    hipLaunchKernelGGL(cidx_fill, dim3((unsigned)(nb * kCidxSeg)), dim3(64), 0, s, nb, D.bands,
                        (const int64_t*)r.tile_base.p, (const TileDesc*)r.tiles.p, kept,
-                       (const int64_t*)c.row_p.p, (int32_t*)c.col.p, (int64_t*)c.off.p,
-                       (int32_t*)c.rows.p, (unsigned long long*)W.stats.p);
+                       (const int32_t*)W.cidx.p, (const int64_t*)c.row_p.p, (int32_t*)c.col.p,
+                       (int64_t*)c.off.p, (int32_t*)c.rows.p, (unsigned long long*)W.stats.p);
   CK(hipGetLastError());
   int64_t ne = 0;
   CK(read_sizes(ctx, W.stats.p, &ne, 1));
