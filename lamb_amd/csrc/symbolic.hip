@@ -170,6 +170,40 @@ __device__ __forceinline__ int rows_of_mask(const int16_t* rs, uint32_t m) {
   return r;
 }
 
+// A wave's walk over its A strips [lo, hi), two strips ahead: k2 is the K
+// atom of the strip after next; k1 / kd1 / b01 / b11 (its block size and B
+// strip range) belong to the next strip, so that the loads for a strip are
+// in flight while the one before is processed.
+struct StripWalk {
+  const Strip* as;
+  const int32_t* blk;
+  const int64_t* b_ptr;
+  int64_t hi;
+  int32_t k1 = 0, k2 = 0, kd1 = 0;
+  int64_t b01 = 0, b11 = 0;
+  __device__ __forceinline__ StripWalk(const Strip* a, const int32_t* bk, const int64_t* bp,
+                                       int64_t lo, int64_t h)
+      : as(a), blk(bk), b_ptr(bp), hi(h) {
+    if (lo < hi) {
+      k1 = as[lo].k;
+      kd1 = blk[k1];
+      b01 = b_ptr[k1];
+      b11 = b_ptr[k1 + 1];
+    }
+    if (lo + 1 < hi) k2 = as[lo + 1].k;
+  }
+  // called while strip sa is processed: make sa + 1 the next strip
+  __device__ __forceinline__ void advance(int64_t sa) {
+    if (sa + 1 < hi) {
+      k1 = k2;
+      kd1 = blk[k1];
+      b01 = b_ptr[k1];
+      b11 = b_ptr[k1 + 1];
+    }
+    if (sa + 2 < hi) k2 = as[sa + 2].k;
+  }
+};
+
 // Calls emit(rm, cm) for every entry of strip pair (SA, SB) under the
 // prefilter nsq(A_ik) * nsq(B_kj) >= eps2; returns the dropped pair count.
 template <typename F>
@@ -295,11 +329,13 @@ __global__ __launch_bounds__(kLds ? 512 : 64) void sym_count(
   };
   unsigned long long kept = 0, dropped = 0, partial = 0;
   double flops = 0.0;
+  StripWalk walk(as, blk, b_ptr, lo, hi);
   for (int64_t sa = lo; sa < hi; ++sa) {
     const Strip* SA = as + sa;
-    const int32_t k = SA->k;
-    const int kd = blk[k];
-    const int64_t b0 = b_ptr[k], b1 = b_ptr[k + 1];
+    const int32_t k = walk.k1;
+    const int kd = walk.kd1;
+    const int64_t b0 = walk.b01, b1 = walk.b11;
+    walk.advance(sa);
     for (int64_t sb = b0 + lane; sb < b1; sb += 64) {
       const Strip* SB = bs + sb;
       const int32_t J = SB->band;
@@ -520,11 +556,14 @@ __global__ __launch_bounds__(kLds ? 512 : 64) void sym_fill(
   }
   int32_t* c = kLds ? scur + (int64_t)g * nbands : cursor + ((int64_t)I * nseg + g) * nbands;
   int32_t* kc = kLds ? scur + nsb + (int64_t)g * nbands : kcur + ((int64_t)I * nseg + g) * nbands;
+  StripWalk walk(as, blk, b_ptr, lo, hi);
   for (int64_t sa = lo; sa < hi; ++sa) {
     const Strip* SA = as + sa;
-    const int32_t k = SA->k;
-    const int kd = blk[k];
-    const int64_t b0 = b_ptr[k], b1 = b_ptr[k + 1];
+    const int32_t k = walk.k1;
+    const int kd = walk.kd1;
+    const int64_t b0 = walk.b01, b1 = walk.b11;
+    walk.advance(sa);
+    (void)k;
     for (int64_t sb = b0 + lane; sb < b1; sb += 64) {
       const Strip* SB = bs + sb;
       const int32_t J = SB->band;
