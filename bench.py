@@ -22,7 +22,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per numeric_f64_kernel launch on this workload: rocprofv3
 # FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, the factors read off a device copy
 # of known size in the same run (profiles/numeric_f64_kernel_water6144.md).
-PROFILED_TRAFFIC_BYTES = 121.58e9
+PROFILED_TRAFFIC_BYTES = 121.42e9
 
 
 def parse():
