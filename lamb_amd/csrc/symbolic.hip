@@ -2,6 +2,9 @@
 // tiles and their (A strip, B strip) entries, and the C index after the
 // numeric kernel. Every step is deterministic: entries of a tile are in
 // ascending K atom order and C blocks in ascending column order.
+#include <cstdlib>
+#include <cstring>
+
 #include "common.h"
 #include "device_util.h"
 #include "symbolic.h"
@@ -411,6 +414,12 @@ static size_t sym_fill_lds(int32_t nbands, int nseg) {
   return 2 * sizeof(int32_t) * (size_t)nseg * (size_t)nbands;
 }
 constexpr size_t kSymLdsMax = 96 * 1024;
+// LAMB_HIP_SYM_LDS=0 takes the global-memory forms of sym_count / sym_fill
+// whatever the band count (they serve more than kSymLdsMax bytes per band)
+static bool sym_lds_allowed() {
+  const char* v = getenv("LAMB_HIP_SYM_LDS");
+  return !(v && strcmp(v, "0") == 0);
+}
 
 // Per band: tiles present, their entry counts and C sizes (lanes over J).
 __global__ __launch_bounds__(64) void sym_band_totals(
@@ -854,7 +863,7 @@ hipError_t build_tiles(lamb_hip_ctx* ctx, SymScratch& W, const DistView& D, cons
   CK(W.stats.ensure(sizeof(unsigned long long) * 16));
   // a band's pattern words and counters in LDS when they fit
   const size_t lds_cnt = sym_count_lds(nb, nseg);
-  const bool cnt_lds = lds_cnt <= kSymLdsMax;
+  const bool cnt_lds = lds_cnt <= kSymLdsMax && sym_lds_allowed();
   if (!cnt_lds) {
     CK(fill32(s, (int32_t*)W.cnt.p, dense, 0));
     CK(fill32(s, (int32_t*)W.ksum.p, dense, 0));
@@ -917,7 +926,7 @@ hipError_t build_tiles(lamb_hip_ctx* ctx, SymScratch& W, const DistView& D, cons
   CK(hipGetLastError());
   if (nb) {
     const size_t lds_fill = sym_fill_lds(nb, nseg);
-    if (lds_fill <= kSymLdsMax)
+    if (lds_fill <= kSymLdsMax && sym_lds_allowed())
       hipLaunchKernelGGL(sym_fill<true>, dim3((unsigned)nb), dim3(64 * nseg), lds_fill, s, nb,
                          nseg, D.blk, (const Strip*)A.strips.p, (const int64_t*)A.ptr.p,
                          (const Strip*)B.strips.p, (const int64_t*)B.ptr.p,

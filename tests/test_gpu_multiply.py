@@ -209,3 +209,12 @@ def test_forced_variants_whole_and_partial_products(ctx, monkeypatch, variant):
     b = S.decay_matrix(atoms, 9.0, 1.3, 32)
     st, rst = check(ctx, atoms, a, b, eps=2e-3)
     assert rst["n_triples_filtered"] > 0
+
+
+@pytest.mark.parametrize("case", ["water_eps", "filters_bind", "alpha_beta", "three_kinds"])
+def test_symbolic_global_memory_forms(ctx, monkeypatch, case):
+    """The global-memory forms of the symbolic kernels (taken for systems with
+    more bands than fit the per-band LDS buffers) against the oracle."""
+    monkeypatch.setenv("LAMB_HIP_SYM_LDS", "0")
+    {"water_eps": test_water_eps, "filters_bind": test_filters_bind,
+     "alpha_beta": test_alpha_beta, "three_kinds": test_three_kinds}[case](ctx)
