@@ -58,27 +58,30 @@ __global__ void fill_exp_kernel(int32_t* __restrict__ ex, int64_t n) {
   if (i < n) ex[i] = kNoExp;
 }
 
-// One thread per stored block: the largest exponent of each of its rows
-// (row_side: A, rows of the matrix) or columns (B) into ex by atomicMax.
+// Sixteen lanes per stored block, one per row (row_side: A, rows of the
+// matrix) or column (B) of it (and rows / columns l + 16, l + 32, ... of
+// larger blocks): the largest exponent of that line into ex by atomicMax.
+// For A the sixteen lanes read consecutive elements of a block column.
 __global__ __launch_bounds__(256) void slice_exp_kernel(
     int64_t nblk, const int32_t* __restrict__ rows, const int32_t* __restrict__ cols,
     const int32_t* __restrict__ blk, const int64_t* __restrict__ elem0,
     const int64_t* __restrict__ off, const double* __restrict__ data, int row_side,
     int32_t* __restrict__ ex) {
   LAMB_GUARD;
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t b = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+  const int l = threadIdx.x & 15;
   if (b >= nblk) return;
   const int32_t ra = rows[b], ca = cols[b];
   const int m = blk[ra], n = blk[ca];
   const double* x = data + off[b];
   if (row_side) {
-    for (int i = 0; i < m; ++i) {
+    for (int i = l; i < m; i += 16) {
       int32_t e = kNoExp;
       for (int j = 0; j < n; ++j) e = max(e, exp_above(x[(int64_t)j * m + i]));
       if (e != kNoExp) atomicMax(&ex[elem0[ra] + i], e);
     }
   } else {
-    for (int j = 0; j < n; ++j) {
+    for (int j = l; j < n; j += 16) {
       int32_t e = kNoExp;
       for (int i = 0; i < m; ++i) e = max(e, exp_above(x[(int64_t)j * m + i]));
       if (e != kNoExp) atomicMax(&ex[elem0[ca] + j], e);
@@ -363,7 +366,7 @@ hipError_t launch_slice_exponents(hipStream_t s, int64_t nblk, const int32_t* ro
   if (n_elem > 0)
     hipLaunchKernelGGL(fill_exp_kernel, dim3((unsigned)grid1(n_elem)), dim3(256), 0, s, ex, n_elem);
   if (nblk > 0)
-    hipLaunchKernelGGL(slice_exp_kernel, dim3((unsigned)grid1(nblk)), dim3(256), 0, s, nblk, rows,
+    hipLaunchKernelGGL(slice_exp_kernel, dim3((unsigned)grid1(nblk * 16)), dim3(256), 0, s, nblk, rows,
                        cols, blk, elem0, off, data, row_side ? 1 : 0, ex);
   return hipGetLastError();
 }

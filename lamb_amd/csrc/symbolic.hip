@@ -421,19 +421,22 @@ static bool sym_lds_allowed() {
   return !(v && strcmp(v, "0") == 0);
 }
 
-// Per band: tiles present, their entry counts and C sizes (lanes over J).
-__global__ __launch_bounds__(64) void sym_band_totals(
-    int32_t nbands, int nseg, const Band* __restrict__ bands, const int32_t* __restrict__ cnt,
-    const uint32_t* __restrict__ pat, int64_t* __restrict__ band_tiles,
-    int64_t* __restrict__ band_entries, int64_t* __restrict__ band_csize,
-    int64_t* __restrict__ band_cblocks) {
+// Per (row band I, chunk of 64 column bands): tiles present, their entry
+// counts, C sizes and C blocks (lanes over J). One wave per (I, chunk); the
+// totals of (I, c) go to slot I * nchunk + c of each of the four arrays.
+__global__ __launch_bounds__(64) void sym_chunk_totals(
+    int32_t nbands, int nseg, int32_t nchunk, const Band* __restrict__ bands,
+    const int32_t* __restrict__ cnt, const uint32_t* __restrict__ pat,
+    int64_t* __restrict__ ch_tiles, int64_t* __restrict__ ch_entries,
+    int64_t* __restrict__ ch_csize, int64_t* __restrict__ ch_cblocks) {
   LAMB_GUARD;
-  const int32_t I = blockIdx.x;
+  const int32_t I = (int32_t)(blockIdx.x / nchunk), c = (int32_t)(blockIdx.x % nchunk);
   if (I >= nbands) return;
   const int lane = threadIdx.x;
   const Band& bI = bands[I];
   int64_t nt = 0, ne = 0, cs = 0, cb = 0;
-  for (int32_t J = lane; J < nbands; J += 64) {
+  const int32_t J = c * 64 + lane;
+  if (J < nbands) {
     int64_t e = 0;
     for (int g = 0; g < nseg; ++g) e += cnt[((int64_t)I * nseg + g) * nbands + J];
     const uint32_t* pt = pat + ((int64_t)I * nbands + J) * (kBandAtoms / 2);
@@ -451,8 +454,8 @@ __global__ __launch_bounds__(64) void sym_band_totals(
         }
     }
     if (any || e) {
-      ++nt;
-      ne += e;
+      nt = 1;
+      ne = e;
     }
   }
   nt = wave_sum_i64(nt);
@@ -460,27 +463,41 @@ __global__ __launch_bounds__(64) void sym_band_totals(
   cs = wave_sum_i64(cs);
   cb = wave_sum_i64(cb);
   if (lane == 0) {
-    band_tiles[I] = nt;
-    band_entries[I] = ne;
-    band_csize[I] = cs;
-    band_cblocks[I] = cb;
+    const int64_t x = (int64_t)I * nchunk + c;
+    ch_tiles[x] = nt;
+    ch_entries[x] = ne;
+    ch_csize[x] = cs;
+    ch_cblocks[x] = cb;
   }
 }
 
+// tile_base[I] = first tile of band I (the scanned tile counts of its first
+// chunk); tile_base[nbands] = all tiles
+__global__ void band_tile_base(int32_t nbands, int32_t nchunk, const int64_t* __restrict__ scanned,
+                               int64_t* __restrict__ tile_base) {
+  LAMB_GUARD;
+  const int32_t I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I <= nbands) tile_base[I] = scanned[(int64_t)I * nchunk];
+}
+
 // Tile descriptors (J ascending within a band); turns cnt/ksum into the
-// per-(I, g, J) entry cursors and K-column starts used by sym_fill.
+// per-(I, g, J) entry cursors and K-column starts used by sym_fill. One wave
+// per (I, chunk of 64 column bands), starting at the chunk's scanned tile,
+// entry and C-size offsets.
 __global__ __launch_bounds__(64) void sym_tiles(
-    int32_t nbands, int nseg, const Band* __restrict__ bands, int32_t* __restrict__ cnt,
-    int32_t* __restrict__ ksum, const uint32_t* __restrict__ pat,
+    int32_t nbands, int nseg, int32_t nchunk, const Band* __restrict__ bands,
+    int32_t* __restrict__ cnt, int32_t* __restrict__ ksum, const uint32_t* __restrict__ pat,
     const int64_t* __restrict__ tile_base, const int64_t* __restrict__ entry_base,
     const int64_t* __restrict__ csize_base, TileDesc* __restrict__ tiles) {
   LAMB_GUARD;
-  const int32_t I = blockIdx.x;
+  const int32_t I = (int32_t)(blockIdx.x / nchunk), c = (int32_t)(blockIdx.x % nchunk);
   if (I >= nbands) return;
   const int lane = threadIdx.x;
   const Band& bI = bands[I];
-  int64_t t_run = tile_base[I], e_run = entry_base[I], c_run = csize_base[I];
-  for (int32_t J0 = 0; J0 < nbands; J0 += 64) {
+  const int64_t x0 = (int64_t)I * nchunk + c;
+  int64_t t_run = tile_base[x0], e_run = entry_base[x0], c_run = csize_base[x0];
+  {
+    const int32_t J0 = c * 64;
     const int32_t J = J0 + lane;
     int64_t e = 0, k = 0, cs = 0;
     bool any = false;
@@ -755,8 +772,9 @@ __global__ void tile_perm(int64_t ntiles, int rb, int32_t nbands, const TileDesc
   perm[base[c] + __builtin_popcount(below)] = (int32_t)t;
 }
 
-// out[0..3] = the four band-total sums (after their scans, at [v*(nb+2)+nb]),
-// out[4..7] = kept, dropped, flops and partial-entry statistics
+// out[0..3] = the four chunk-total sums (after their scans, at [v*(nb+2)+nb]
+// with nb the number of scanned values), out[4..7] = kept, dropped, flops
+// and partial-entry statistics
 __global__ void gather_totals(const int64_t* __restrict__ bts, int32_t nb,
                               const int64_t* __restrict__ stats, int64_t* __restrict__ out) {
   LAMB_GUARD;
@@ -887,22 +905,25 @@ hipError_t build_tiles(lamb_hip_ctx* ctx, SymScratch& W, const DistView& D, cons
                        (uint32_t*)W.pat.p, (unsigned long long*)W.stats.p, mp);
   }
   CK(hipGetLastError());
-  // per-band totals and their scans
-  CK(W.band4.ensure(sizeof(int64_t) * 4 * (size_t)(nb + 1)));
-  CK(W.band4s.ensure(sizeof(int64_t) * 4 * (size_t)(nb + 2)));
+  // per-(band, 64-band chunk) totals, scanned in (band, chunk) order: each
+  // chunk's scanned value is where its tiles, entries and C blocks start
+  const int32_t nch = std::max<int32_t>((nb + 63) / 64, 1);
+  const int64_t nx = (int64_t)nb * nch;
+  CK(W.band4.ensure(sizeof(int64_t) * 4 * (size_t)(nx + 1)));
+  CK(W.band4s.ensure(sizeof(int64_t) * 4 * (size_t)(nx + 2)));
   int64_t* bt = (int64_t*)W.band4.p;
   int64_t* bts = (int64_t*)W.band4s.p;
   if (nb)
-    hipLaunchKernelGGL(sym_band_totals, dim3(nb), dim3(64), 0, s, nb, nseg, D.bands,
-                       (const int32_t*)W.cnt.p, (const uint32_t*)W.pat.p, bt, bt + (nb + 1),
-                       bt + 2 * (nb + 1), bt + 3 * (nb + 1));
+    hipLaunchKernelGGL(sym_chunk_totals, dim3((unsigned)nx), dim3(64), 0, s, nb, nseg, nch,
+                       D.bands, (const int32_t*)W.cnt.p, (const uint32_t*)W.pat.p, bt,
+                       bt + (nx + 1), bt + 2 * (nx + 1), bt + 3 * (nx + 1));
   CK(hipGetLastError());
-  CK(W.scan.ensure(sizeof(int64_t) * (size_t)scan_scratch_elems(nb + 1)));
+  CK(W.scan.ensure(sizeof(int64_t) * (size_t)scan_scratch_elems(nx + 1)));
   for (int v = 0; v < 4; ++v)
-    CK(exclusive_scan(s, bt + v * (nb + 1), bts + v * (nb + 2), nb, (int64_t*)W.scan.p));
+    CK(exclusive_scan(s, bt + v * (nx + 1), bts + v * (nx + 2), nx, (int64_t*)W.scan.p));
   // the four totals and four statistics, gathered for one read
   int64_t* gathered = (int64_t*)W.stats.p + 8;
-  hipLaunchKernelGGL(gather_totals, dim3(1), dim3(64), 0, s, bts, nb,
+  hipLaunchKernelGGL(gather_totals, dim3(1), dim3(64), 0, s, bts, (int32_t)nx,
                      (const int64_t*)W.stats.p, gathered);
   CK(hipGetLastError());
   int64_t tot[8];
@@ -918,11 +939,12 @@ hipError_t build_tiles(lamb_hip_ctx* ctx, SymScratch& W, const DistView& D, cons
   CK(r.tiles.ensure(sizeof(TileDesc) * (size_t)std::max<int64_t>(r.n_tiles, 1)));
   CK(r.entries.ensure(sizeof(Entry) * (size_t)std::max<int64_t>(r.n_entries, 1)));
   CK(r.tile_base.ensure(sizeof(int64_t) * (size_t)(nb + 1)));
-  CK(hipMemcpyAsync(r.tile_base.p, bts, sizeof(int64_t) * (size_t)(nb + 1), hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(band_tile_base, dim3(grid_of(nb + 1)), dim3(256), 0, s, nb, nch,
+                     (const int64_t*)bts, (int64_t*)r.tile_base.p);
   if (nb)
-    hipLaunchKernelGGL(sym_tiles, dim3(nb), dim3(64), 0, s, nb, nseg, D.bands, (int32_t*)W.cnt.p,
-                       (int32_t*)W.ksum.p, (const uint32_t*)W.pat.p, bts, bts + (nb + 2),
-                       bts + 2 * (nb + 2), (TileDesc*)r.tiles.p);
+    hipLaunchKernelGGL(sym_tiles, dim3((unsigned)nx), dim3(64), 0, s, nb, nseg, nch, D.bands,
+                       (int32_t*)W.cnt.p, (int32_t*)W.ksum.p, (const uint32_t*)W.pat.p, bts,
+                       bts + (nx + 2), bts + 2 * (nx + 2), (TileDesc*)r.tiles.p);
   CK(hipGetLastError());
   if (nb) {
     const size_t lds_fill = sym_fill_lds(nb, nseg);
