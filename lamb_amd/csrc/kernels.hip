@@ -80,19 +80,20 @@ __global__ __launch_bounds__(256) void pack_strips_kernel(int64_t n_strips,
   const int64_t o = have ? strips[s].off[pos] : 0;
   const int in = lane - lo, ext = hi - lo;
   double* out = packed + pk_off[s];
-  // 8 K columns per pass: the loads of a pass are independent and in flight
-  // together before the stores
-  for (int k0 = 0; k0 < kdim; k0 += 8) {
-    double v[8];
+  // 16 K columns per pass (every 5- and 13-row atom in one pass): the loads
+  // of a pass are independent and in flight together before the stores
+  constexpr int kPass = 16;
+  for (int k0 = 0; k0 < kdim; k0 += kPass) {
+    double v[kPass];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < kPass; ++j) {
       const int kk = k0 + j;
       v[j] = 0.0;
       if (have && kk < kdim)
         v[j] = row_side ? data[o + (int64_t)kk * ext + in] : data[o + (int64_t)in * kdim + kk];
     }
 #pragma unroll
-    for (int j = 0; j < 8; ++j)
+    for (int j = 0; j < kPass; ++j)
       if (k0 + j < kdim) out[(k0 + j) * 64 + lane] = v[j];
   }
 }
