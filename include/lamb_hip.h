@@ -139,6 +139,23 @@ int32_t lamb_hip_sector_query(lamb_hip_mat* mat, int32_t subset_row, int32_t sub
 int32_t lamb_hip_sector_download(lamb_hip_mat* mat, int32_t subset_row, int32_t subset_col,
                                  int64_t* row_p, int16_t* col_i, int64_t* extents,
                                  double* data);
+/* Sector wire format (lamb_goods_X_create, src/lamb_goods_X_types.F90:119-268;
+ * lamb_goods_X_unpack, src/lamb_goods_X_methods.F90:166-218): an int64 index
+ * array (12 header words of 1-based offsets and sizes, then row_p's first
+ * n_rows entries, col_i as int16 packed little-endian into words, extents as
+ * (lo, hi) pairs, one zero flag byte per block; each array max(1, ceil(bytes/8))
+ * words) and the sector's FP64 data. goods_size gives both lengths,
+ * goods_pack fills caller buffers of exactly those lengths, goods_unpack
+ * validates the header against subset_row and stages the sector like
+ * lamb_hip_sector_upload (lamb_hip_matrix_commit makes it live). */
+int32_t lamb_hip_goods_size(lamb_hip_mat* mat, int32_t subset_row, int32_t subset_col,
+                            int64_t* n_index_words, int64_t* n_data);
+int32_t lamb_hip_goods_pack(lamb_hip_mat* mat, int32_t subset_row, int32_t subset_col,
+                            int64_t* index, int64_t n_index_words, double* data,
+                            int64_t n_data);
+int32_t lamb_hip_goods_unpack(lamb_hip_mat* mat, int32_t subset_row, int32_t subset_col,
+                              const int64_t* index, int64_t n_index_words,
+                              const double* data, int64_t n_data);
 /* Totals of the whole matrix. */
 int32_t lamb_hip_matrix_info(const lamb_hip_mat* mat, int64_t* n_blks, int64_t* n_elems);
 

@@ -75,6 +75,11 @@ SIGNATURES = {
     "lamb_hip_sector_download": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, i64p, i16p,
                                              i64p, f64p]),
     "lamb_hip_matrix_info": (C.c_int32, [C.c_void_p, i64p, i64p]),
+    "lamb_hip_goods_size": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, i64p, i64p]),
+    "lamb_hip_goods_pack": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, i64p, C.c_int64,
+                                        f64p, C.c_int64]),
+    "lamb_hip_goods_unpack": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, i64p, C.c_int64,
+                                          f64p, C.c_int64]),
     "lamb_hip_multiply": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_double, C.c_double, C.c_double, C.c_int32,
                                       C.POINTER(Stats)]),

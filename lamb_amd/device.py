@@ -154,6 +154,34 @@ class DeviceMatrix:
             sectors.append(Sector((r, s), row_p, col_i, ext, data))
         return LambMatrix(dist, sectors)
 
+    def goods_pack(self, subset_row: int, subset_col: int):
+        """One sector in the wire format of lamb_goods_X_create
+        (src/lamb_goods_X_types.F90:119-268): (index int64 words, data f64)."""
+        lib = self.ctx.lib
+        ni, nd = C.c_int64(), C.c_int64()
+        self.ctx.check(lib.lamb_hip_goods_size(self.h, subset_row, subset_col, C.byref(ni),
+                                               C.byref(nd)), "goods_size")
+        index = np.zeros(ni.value, np.int64)
+        data = np.zeros(nd.value, np.float64)
+        self.ctx.check(lib.lamb_hip_goods_pack(self.h, subset_row, subset_col,
+                                               N.ptr(index, C.c_int64), len(index),
+                                               N.ptr(data, C.c_double), len(data)),
+                       f"goods_pack({subset_row},{subset_col})")
+        return index, data
+
+    def goods_unpack(self, subset_row: int, subset_col: int, index, data, commit=True):
+        """Stage one sector from the wire format (lamb_goods_X_unpack,
+        src/lamb_goods_X_methods.F90:166-218); commit makes it live."""
+        lib = self.ctx.lib
+        index = np.ascontiguousarray(index, dtype=np.int64)
+        data = np.ascontiguousarray(data, dtype=np.float64)
+        self.ctx.check(lib.lamb_hip_goods_unpack(self.h, subset_row, subset_col,
+                                                 N.ptr(index, C.c_int64), len(index),
+                                                 N.ptr(data, C.c_double), len(data)),
+                       f"goods_unpack({subset_row},{subset_col})")
+        if commit:
+            self.ctx.check(lib.lamb_hip_matrix_commit(self.h), "matrix_commit")
+
     def copy_from(self, other: "DeviceMatrix"):
         self.ctx.check(self.ctx.lib.lamb_hip_matrix_copy(other.h, self.h), "matrix_copy")
 

@@ -50,6 +50,7 @@ module lamb_hip
   public :: lamb_hip_matrix_create, lamb_hip_matrix_destroy, lamb_hip_matrix_copy
   public :: lamb_hip_sector_upload, lamb_hip_distribute, lamb_hip_matrix_commit
   public :: lamb_hip_sector_query, lamb_hip_sector_download, lamb_hip_matrix_info
+  public :: lamb_hip_goods_size, lamb_hip_goods_pack, lamb_hip_goods_unpack
   public :: lamb_hip_multiply, lamb_hip_trace, lamb_hip_dbcsr_load, lamb_hip_partition_atoms
   public :: lamb_multiply_hip, internal_distribute_hip, lamb_load_dbcsr_hip
 
@@ -168,6 +169,42 @@ module lamb_hip
       integer(c_int16_t), intent(out) :: col_i(*)
       integer(c_int64_t), intent(out) :: extents(*)
       real(c_double), intent(out) :: data(*)
+      integer(c_int32_t) :: st
+    end function
+
+    ! Sector wire format of lamb_goods_X_create / lamb_goods_X_unpack
+    ! (src/lamb_goods_X_types.F90:119-268, src/lamb_goods_X_methods.F90:166-218):
+    ! index holds goods_X%index, data goods_X%data.
+    function lamb_hip_goods_size(mat, subset_row, subset_col, n_index_words, n_data) &
+        bind(C, name="lamb_hip_goods_size") result(st)
+      import :: c_int32_t, c_int64_t, c_ptr
+      type(c_ptr), value :: mat
+      integer(c_int32_t), value :: subset_row, subset_col
+      integer(c_int64_t), intent(out) :: n_index_words, n_data
+      integer(c_int32_t) :: st
+    end function
+
+    function lamb_hip_goods_pack(mat, subset_row, subset_col, index, n_index_words, data, &
+        n_data) bind(C, name="lamb_hip_goods_pack") result(st)
+      import :: c_int32_t, c_int64_t, c_double, c_ptr
+      type(c_ptr), value :: mat
+      integer(c_int32_t), value :: subset_row, subset_col
+      integer(c_int64_t), intent(out) :: index(*)
+      integer(c_int64_t), value :: n_index_words
+      real(c_double), intent(out) :: data(*)
+      integer(c_int64_t), value :: n_data
+      integer(c_int32_t) :: st
+    end function
+
+    function lamb_hip_goods_unpack(mat, subset_row, subset_col, index, n_index_words, data, &
+        n_data) bind(C, name="lamb_hip_goods_unpack") result(st)
+      import :: c_int32_t, c_int64_t, c_double, c_ptr
+      type(c_ptr), value :: mat
+      integer(c_int32_t), value :: subset_row, subset_col
+      integer(c_int64_t), intent(in) :: index(*)
+      integer(c_int64_t), value :: n_index_words
+      real(c_double), intent(in) :: data(*)
+      integer(c_int64_t), value :: n_data
       integer(c_int32_t) :: st
     end function
 

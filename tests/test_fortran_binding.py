@@ -43,6 +43,9 @@ program link_check
     st = lamb_hip_sector_query(a, 1_c_int32_t, 1_c_int32_t, nb, ne)
     st = lamb_hip_sector_download(a, 1_c_int32_t, 1_c_int32_t, row_p, col, ext, d)
     st = lamb_hip_matrix_info(a, nb, ne)
+    st = lamb_hip_goods_size(a, 1_c_int32_t, 1_c_int32_t, nb, ne)
+    st = lamb_hip_goods_pack(a, 1_c_int32_t, 1_c_int32_t, row_p, 2_c_int64_t, d, 1_c_int64_t)
+    st = lamb_hip_goods_unpack(a, 1_c_int32_t, 1_c_int32_t, row_p, 2_c_int64_t, d, 1_c_int64_t)
     st = lamb_hip_matrix_copy(a, a)
     st = lamb_hip_multiply(ctx, a, a, a, 1.0_c_double, 0.0_c_double, 0.0_c_double, 0_c_int32_t, stats)
     st = lamb_hip_trace(a, tr)
