@@ -314,6 +314,13 @@ int32_t lamb_hip_dbcsr_read_blocks(const char* path, int64_t n_blocks, int32_t* 
  * unimplemented, :409-412) and square over the distribution's atoms, with
  * its block sizes. */
 int32_t lamb_hip_dbcsr_load(lamb_hip_mat* mat, const char* path);
+/* lamb_multiply_test (src/lamb_multiplication.F90:85-114): the DBCSR dump at
+ * path loaded as A (lamb_load_dbcsr), copied to B and C (lamb_matrix_copy),
+ * then C = A*B with eps_filter, FP64; stats may be NULL. The distribution
+ * is the file's atoms in file order, in subsets of at most 32,767 atoms.
+ * All three matrices are released before the call returns. */
+int32_t lamb_hip_multiply_test(lamb_hip_ctx* ctx, const char* path, double eps_filter,
+                               lamb_hip_stats* stats);
 
 #ifdef __cplusplus
 }

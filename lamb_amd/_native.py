@@ -113,6 +113,8 @@ SIGNATURES = {
     "lamb_hip_dbcsr_read_blocks": (C.c_int32, [C.c_char_p, C.c_int64, i32p, i32p, C.c_int64,
                                                f64p, C.c_char_p, C.c_int32]),
     "lamb_hip_dbcsr_load": (C.c_int32, [C.c_void_p, C.c_char_p]),
+    "lamb_hip_multiply_test": (C.c_int32, [C.c_void_p, C.c_char_p, C.c_double,
+                                           C.POINTER(Stats)]),
     "lamb_hip_partition_atoms": (C.c_int32, [C.c_void_p, C.c_int64, f32p, f32p, i32p, i32p,
                                              i16p]),
 }

@@ -338,4 +338,55 @@ int32_t lamb_hip_dbcsr_load(lamb_hip_mat* m, const char* path) {
   }
 }
 
+// lamb_multiply_test (src/lamb_multiplication.F90:85-114): load a DBCSR
+// dump as A, copy it to B and C, and multiply C = A*B. The distribution is
+// the file's atoms in file order, split into subsets of at most 32,767 atoms
+// (the int16 local index); the reference takes it from a coordinate file.
+int32_t lamb_hip_multiply_test(lamb_hip_ctx* ctx, const char* path, double eps_filter,
+                               lamb_hip_stats* stats) {
+  if (!ctx) return LAMB_HIP_FAILURE;
+  if (!path)
+    return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_WRONG_ARGS, "multiply_test: no path");
+  std::vector<int8_t> blk;
+  try {
+    Reader R;
+    Layout L;
+    if (!R.open(path) || !parse(R, L))
+      return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_EXTERNAL_ERROR, "%s", R.err.c_str());
+    if (L.nr != L.nc || L.rbs != L.cbs)
+      return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_WRONG_ARGS,
+                       "multiply_test: %s is not square in atoms (%d x %d block rows / "
+                       "columns or different sizes)",
+                       path, L.nr, L.nc);
+    blk.assign(L.rbs.begin(), L.rbs.end());
+  } catch (const std::bad_alloc&) {
+    return set_error(ctx, LAMB_HIP_MEMORY, LAMB_HIP_EXTERNAL_ERROR,
+                     "multiply_test: out of host memory");
+  }
+  const int64_t n = (int64_t)blk.size();
+  constexpr int64_t kSubsetMax = 32767;
+  const int32_t n_subsets = (int32_t)std::max<int64_t>(1, (n + kSubsetMax - 1) / kSubsetMax);
+  std::vector<int32_t> smap(n);
+  std::vector<int16_t> omap(n);
+  for (int64_t g = 0; g < n; ++g) {
+    smap[g] = (int32_t)(g / kSubsetMax) + 1;
+    omap[g] = (int16_t)(g % kSubsetMax + 1);
+  }
+  lamb_hip_dist* dist = nullptr;
+  lamb_hip_mat* mats[3] = {nullptr, nullptr, nullptr};
+  int32_t rc = lamb_hip_dist_create(ctx, n, blk.data(), n_subsets, smap.data(), omap.data(),
+                                    nullptr, nullptr, &dist);
+  for (int i = 0; i < 3 && rc == LAMB_HIP_OK; ++i) rc = lamb_hip_matrix_create(dist, &mats[i]);
+  if (rc == LAMB_HIP_OK) rc = lamb_hip_dbcsr_load(mats[0], path);
+  if (rc == LAMB_HIP_OK) rc = lamb_hip_matrix_copy(mats[0], mats[1]);
+  if (rc == LAMB_HIP_OK) rc = lamb_hip_matrix_copy(mats[0], mats[2]);
+  if (rc == LAMB_HIP_OK)
+    rc = lamb_hip_multiply(ctx, mats[0], mats[1], mats[2], 1.0, 0.0, eps_filter, LAMB_HIP_FP64,
+                           stats);
+  for (int i = 2; i >= 0; --i)
+    if (mats[i]) lamb_hip_matrix_destroy(mats[i]);
+  if (dist) lamb_hip_dist_destroy(dist);
+  return rc;
+}
+
 }  // extern "C"

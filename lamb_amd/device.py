@@ -325,3 +325,13 @@ def multiply(a: DeviceMatrix, b: DeviceMatrix, c: DeviceMatrix, *, alpha: float 
                                    float(eps_filter), int(flags), C.byref(st))
     ctx.check(rc, "multiply")
     return st.as_dict()
+
+
+def multiply_test(ctx: Context, path, *, eps_filter: float = 0.0) -> dict:
+    """lamb_multiply_test (src/lamb_multiplication.F90:85-114): the DBCSR dump
+    at path as A, copied to B and C, then C = A*B; returns the statistics."""
+    st = N.Stats()
+    rc = ctx.lib.lamb_hip_multiply_test(ctx.h, os.fsencode(os.fspath(path)), float(eps_filter),
+                                        C.byref(st))
+    ctx.check(rc, "multiply_test")
+    return st.as_dict()

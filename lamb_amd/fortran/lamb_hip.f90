@@ -53,6 +53,7 @@ module lamb_hip
   public :: lamb_hip_goods_size, lamb_hip_goods_pack, lamb_hip_goods_unpack
   public :: lamb_hip_multiply, lamb_hip_trace, lamb_hip_dbcsr_load, lamb_hip_partition_atoms
   public :: lamb_multiply_hip, internal_distribute_hip, lamb_load_dbcsr_hip
+  public :: lamb_hip_multiply_test, lamb_multiply_test_hip
 
   interface
     function lamb_hip_init(device, ctx) bind(C, name="lamb_hip_init") result(st)
@@ -169,6 +170,16 @@ module lamb_hip
       integer(c_int16_t), intent(out) :: col_i(*)
       integer(c_int64_t), intent(out) :: extents(*)
       real(c_double), intent(out) :: data(*)
+      integer(c_int32_t) :: st
+    end function
+
+    function lamb_hip_multiply_test(ctx, path, eps_filter, stats) &
+        bind(C, name="lamb_hip_multiply_test") result(st)
+      import :: c_int32_t, c_double, c_char, c_ptr, lamb_hip_stats
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: path(*)
+      real(c_double), value :: eps_filter
+      type(lamb_hip_stats), intent(out) :: stats
       integer(c_int32_t) :: st
     end function
 
@@ -319,6 +330,24 @@ contains
     integer(c_int32_t) :: st
     st = lamb_hip_dbcsr_load(matrix%mat, trim(filename)//c_null_char)
     call fetch_error(matrix%ctx, st, error)
+  end subroutine
+
+  ! lamb_multiply_test (src/lamb_multiplication.F90:85-114) without the MPI
+  ! environment: the dump at filename as A, copied to B and C, C = A*B.
+  subroutine lamb_multiply_test_hip(ctx, filename, error, eps_filter, stats)
+    type(c_ptr), intent(in) :: ctx
+    character(len=*), intent(in) :: filename
+    type(lamb_hip_error), intent(inout) :: error
+    real(c_double), intent(in), optional :: eps_filter
+    type(lamb_hip_stats), intent(out), optional :: stats
+    type(lamb_hip_stats) :: st_local
+    real(c_double) :: eps
+    integer(c_int32_t) :: st
+    eps = 0.0_c_double
+    if (present(eps_filter)) eps = eps_filter
+    st = lamb_hip_multiply_test(ctx, trim(filename)//c_null_char, eps, st_local)
+    call fetch_error(ctx, st, error)
+    if (present(stats)) stats = st_local
   end subroutine
 
 end module lamb_hip

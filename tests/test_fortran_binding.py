@@ -50,6 +50,8 @@ program link_check
     st = lamb_hip_multiply(ctx, a, a, a, 1.0_c_double, 0.0_c_double, 0.0_c_double, 0_c_int32_t, stats)
     st = lamb_hip_trace(a, tr)
     call lamb_multiply_hip(ma, mb, mc, err, 1.0e-8_c_double)
+    call lamb_multiply_test_hip(ctx, 'small.dbcsr', err)
+    st = lamb_hip_multiply_test(ctx, 'small.dbcsr'//c_null_char, 0.0_c_double, stats)
     call internal_distribute_hip(ma, 1_c_int64_t, sidx, lidx, ext2, d, err)
     st = lamb_hip_matrix_destroy(a)
     st = lamb_hip_dist_destroy(dist)

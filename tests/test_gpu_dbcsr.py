@@ -81,3 +81,27 @@ def test_symmetric_type_and_wrong_sizes_are_refused(ctx, tmp_path):
         DV.DeviceMatrix.from_dbcsr(dd, tmp_path / "o.dbcsr")
     assert e.value.type == 100
     dd.close()
+
+
+@pytest.mark.parametrize("eps", [0.0, 1e-6])
+def test_multiply_test_entry(ctx, tmp_path, eps):
+    """lamb_hip_multiply_test: dump -> A, copies B and C, C = A*B. Its
+    statistics equal the oracle's for A*A (kept and filtered triples, flops,
+    stored C blocks); the fixture file multiplies too."""
+    atoms = S.water_system(150, seed=51, heavy_fraction=0.1)
+    a = S.decay_matrix(atoms, 7.0, 0.4, 52)
+    D.write(tmp_path / "small.dbcsr", a, n_chunks=3)
+    print(f"case: multiply_test atoms={atoms.n_atoms} eps={eps}", flush=True)
+    st = DV.multiply_test(ctx, tmp_path / "small.dbcsr", eps_filter=eps)
+    ref, rst = oracle.multiply(a, a, eps=eps)
+    assert st["n_triples_kept"] == rst["n_triples_kept"]
+    assert st["n_triples_filtered"] == rst["n_triples_filtered"]
+    assert st["flops"] == rst["flops"]
+    assert st["n_c_blocks_kept"] == ref.n_blocks
+    st = DV.multiply_test(ctx, GOLDEN / "two_chunks.dbcsr")
+    fixture = D.read(GOLDEN / "two_chunks.dbcsr")
+    ref, rst = oracle.multiply(fixture, fixture)
+    assert st["n_c_blocks_kept"] == ref.n_blocks and st["flops"] == rst["flops"]
+    with pytest.raises(DV.LambError) as e:
+        DV.multiply_test(ctx, tmp_path / "missing.dbcsr")
+    assert e.value.type == -2000
