@@ -19,10 +19,14 @@ import numpy as np
 METRIC = "block-SpGEMM GFLOP/s & HBM GB/s: FP64 plain vs BF16x3 batched at ≤1e-9 err"
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per numeric_f64_kernel launch on this workload: rocprofv3
-# FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, the factors read off a device copy
-# of known size in the same run (profiles/numeric_f64_kernel_water6144.md).
-PROFILED_TRAFFIC_BYTES = 121.42e9
+# HBM bytes per launch of the FP64 numeric kernel the autotuner picks on this
+# workload (numeric_f64_half_kernel): rocprofv3 FETCH_SIZE x 2.000 +
+# WRITE_SIZE x 1.000, the factors read off a device copy of known size in the
+# same run (profiles/numeric_f64_half_kernel_water6144.md).
+PROFILED_TRAFFIC_BYTES = 106.84e9
+# FP64 numeric kernel of each autotuner variant (lamb_hip_tuning_info)
+F64_KERNELS = {0: "numeric_f64_kernel", 1: "numeric_f64_ksplit_kernel",
+               2: "numeric_f64_half_kernel"}
 
 
 def parse():
@@ -403,6 +407,7 @@ def main():
     bytes_alg = 8.0 * (elems_a + elems_b + elems_c)
     ms_num = float(np.mean([s["ms_numeric"] for s in sts]))
     achieved_tf = flops / (ms_num * 1e-3) / 1e12
+    variant, _ = D.tuning_info(ctx)
     result = {
         "metric": METRIC,
         "value": flops / (ms_step * 1e-3) / 1e9,
@@ -437,7 +442,7 @@ def main():
         },
         "roofline": {
             "bound": "mfma",
-            "kernel": "numeric_f64_kernel",
+            "kernel": F64_KERNELS.get(variant, str(variant)),
             "achieved": achieved_tf,
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",

@@ -107,11 +107,13 @@ hipError_t launch_pack_strips(hipStream_t s, int64_t n_strips, const Strip* stri
 hipError_t launch_block_nsq(hipStream_t s, int64_t nblk, const int32_t* d_rows,
                             const int32_t* d_cols, const int32_t* d_blk, const int64_t* d_off,
                             const double* d_data, double* d_nsq);
-// FP64 numeric kernel variants: per-wave 32 x 32 quadrants, or whole-tile
-// waves splitting the K steps. whole: the product has no partial entry (the
-// K-split kernel then runs without per-lane masks)
+// FP64 numeric kernel variants: per-wave 32 x 32 quadrants, per-wave 64 x 32
+// halves with two waves splitting each half's K steps, or whole-tile waves
+// splitting the K steps four ways. whole: the product has no partial entry
+// (the half and K-split kernels then run without per-lane masks)
 constexpr int kF64Quadrant = 0;
 constexpr int kF64Ksplit = 1;
+constexpr int kF64Half = 2;
 hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* tiles,
                               const int32_t* perm, const Entry* entries, const Band* bands,
                               const double* a_pack, const double* b_pack, uint32_t a_zero,

@@ -530,6 +530,172 @@ __global__ __launch_bounds__(256, 2) void numeric_f64_ksplit_kernel(
                 kept_out);
 }
 
+// ---------------------------------------------------------------------------
+// FP64 numeric phase, half-tile variant. One workgroup (4 waves) per C tile;
+// wave w owns the 64 x 32 column half c = w & 1 (eight 16 x 16 accumulators)
+// and takes the live steps of that half whose rank has parity p = w >> 1. A
+// step feeds up to 8 MFMAs from 6 loads, and each live step of a half is
+// walked by two waves (the quadrant kernel: up to four, with 4 MFMAs from 4
+// loads). The two partial halves are added in LDS: a two-term sum, so the
+// result does not depend on which wave writes first.
+// ---------------------------------------------------------------------------
+#ifndef LAMB_HALF_PREFETCH
+#define LAMB_HALF_PREFETCH 3
+#endif
+constexpr int kHalfPrefetch = LAMB_HALF_PREFETCH;
+
+template <bool kWhole>
+__global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
+    const TileDesc* __restrict__ tiles, const int32_t* __restrict__ perm,
+    const Entry* __restrict__ entries, const Band* __restrict__ bands,
+    const double* __restrict__ a_pack, const double* __restrict__ b_pack, uint32_t a_zero,
+    uint32_t b_zero, double alpha, double beta, double eps2,
+    const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
+    const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
+    double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
+  LAMB_GUARD;
+  __shared__ double lds[kBandRows * kPitch];
+  uint2* col_ab = reinterpret_cast<uint2*>(lds);
+  uint32_t* col_m = reinterpret_cast<uint32_t*>(col_ab + kTableCols);
+  uint32_t* step_g = col_m + kTableCols;
+  __shared__ TileSmem sm;
+  __shared__ int win_partial;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wc = wave & 1;
+  const uint32_t par = (uint32_t)(wave >> 1);
+  const TileInfo T = tile_setup(tiles, perm, bands, sm);
+  // atom mask bit of this lane's row in row block h / column in column block q
+  uint32_t rbit[4] = {0u, 0u, 0u, 0u}, cbit[2] = {0u, 0u};
+  for (int a = 0; a < T.natI; ++a)
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      const int r = 16 * h + (lane & 15);
+      if (r >= sm.rs_r[a] && r < sm.rs_r[a + 1]) rbit[h] = 1u << a;
+    }
+  for (int a = 0; a < T.natJ; ++a)
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int c = 32 * wc + 16 * q + (lane & 15);
+      if (c >= sm.rs_c[a] && c < sm.rs_c[a + 1]) cbit[q] = 1u << (16 + a);
+    }
+  const int kc = lane >> 4;
+  const double* a_lane = a_pack + (lane & 15);
+  const double* b_lane = b_pack + 32 * wc + (lane & 15);
+  // bits 4h + 2wc + q of a tile block mask: this wave's half
+  const uint32_t half_bits = 0x3333u << (2 * wc);
+
+  dbl4 acc[8];
+#pragma unroll
+  for (int x = 0; x < 8; ++x) acc[x] = dbl4{0.0, 0.0, 0.0, 0.0};
+  uint32_t seen = 0;  // live steps of this half met so far (wave-uniform)
+
+  for (int Q0 = 0; Q0 < T.ktot; Q0 += kTableCols) {
+    const int nsteps = build_step_table<4>(T, sm, entries, Q0, col_ab, col_m, step_g, a_zero,
+                                           b_zero, &win_partial);
+    int cbase = -64;
+    uint64_t live = 0;
+    uint32_t chunk_bm = 0;
+    double ra[8][4], rb[8][2];
+    uint32_t blkmask[8];
+#define HF_FETCH(S)                                                                \
+    {                                                                              \
+      uint32_t bm = 0;                                                             \
+      int st = 0;                                                                  \
+      for (;;) {                                                                   \
+        while (live == 0 && cbase + 64 < nsteps) {                                 \
+          cbase += 64;                                                             \
+          const int sl = cbase + lane;                                             \
+          chunk_bm = tile_blocks(sl < nsteps ? step_g[sl] : 0u) & half_bits;       \
+          live = __ballot(chunk_bm != 0u);                                         \
+        }                                                                          \
+        if (live == 0) break;                                                      \
+        const int l = __builtin_ctzll(live);                                       \
+        live &= live - 1;                                                          \
+        if ((seen++ & 1u) == par) {                                                \
+          st = cbase + l;                                                          \
+          bm = __builtin_amdgcn_readlane(chunk_bm, l);                             \
+          break;                                                                   \
+        }                                                                          \
+      }                                                                            \
+      blkmask[S] = bm;                                                             \
+      const int Q = 4 * st + kc;                                                   \
+      const uint2 ab = col_ab[Q];                                                  \
+      if constexpr (kWhole) {                                                      \
+        /* without a step of this wave the loads read step 0, feeding no MFMA */   \
+        const double* pa = a_lane + (int64_t)ab.x * 64;                            \
+        const double* pb = b_lane + (int64_t)ab.y * 64;                            \
+        _Pragma("unroll") for (int h = 0; h < 4; ++h) ra[S][h] = pa[16 * h];       \
+        _Pragma("unroll") for (int q = 0; q < 2; ++q) rb[S][q] = pb[16 * q];       \
+        __builtin_amdgcn_sched_barrier(0);                                         \
+      } else {                                                                     \
+        const uint32_t cm = col_m[Q];                                              \
+        _Pragma("unroll") for (int h = 0; h < 4; ++h) {                            \
+          const uint32_t need = (bm & (0xFu << (4 * h))) ? rbit[h] : 0u;           \
+          const uint32_t c = (cm & need) ? ab.x : a_zero;                          \
+          ra[S][h] = a_lane[(int64_t)c * 64 + 16 * h];                             \
+        }                                                                          \
+        _Pragma("unroll") for (int q = 0; q < 2; ++q) {                            \
+          const uint32_t need = (bm & (0x1111u << (2 * wc + q))) ? cbit[q] : 0u;   \
+          const uint32_t c = (cm & need) ? ab.y : b_zero;                          \
+          rb[S][q] = b_lane[(int64_t)c * 64 + 16 * q];                             \
+        }                                                                          \
+      }                                                                            \
+    }
+#define HF_MFMA(S)                                                                 \
+    {                                                                              \
+      const uint32_t bm = blkmask[S] >> (2 * wc);                                  \
+      _Pragma("unroll") for (int h = 0; h < 4; ++h)                                \
+        _Pragma("unroll") for (int q = 0; q < 2; ++q)                              \
+          if ((bm >> (4 * h + q)) & 1u)                                            \
+            acc[2 * h + q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[S][h], rb[S][q], acc[2 * h + q], 0, 0, 0); \
+    }
+#define HF_SLOTS(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
+    static_assert(kHalfPrefetch >= 1 && kHalfPrefetch <= 8, "prefetch depth 1..8");
+#define HF_PRIME(S) if (S < kHalfPrefetch) HF_FETCH(S)
+#define HF_ANY(S) if (S < kHalfPrefetch) any |= blkmask[S];
+#define HF_STEP(S) if (S < kHalfPrefetch) { HF_MFMA(S); HF_FETCH(S); }
+    HF_SLOTS(HF_PRIME)
+    for (;;) {
+      uint32_t any = 0;
+      HF_SLOTS(HF_ANY)
+      if (!any) break;
+      HF_SLOTS(HF_STEP)
+    }
+#undef HF_STEP
+#undef HF_ANY
+#undef HF_PRIME
+#undef HF_SLOTS
+#undef HF_FETCH
+#undef HF_MFMA
+  }
+  __syncthreads();
+
+  // ---- epilogue: the two partial halves added in LDS ----
+  // v_mfma_f64_16x16x4 result: D[i][j] in lane 16*(i%4) + j, register i/4.
+  double* Ct = lds;
+  for (int p = 0; p < 2; ++p) {
+    if ((wave >> 1) == p) {
+#pragma unroll
+      for (int h = 0; h < 4; ++h)
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int idx =
+                (16 * h + 4 * r + (lane >> 4)) * kPitch + 32 * wc + 16 * q + (lane & 15);
+            const double v = acc[2 * h + q][r];
+            Ct[idx] = p == 0 ? v : Ct[idx] + v;
+          }
+    }
+    __syncthreads();
+  }
+  tile_epilogue(Ct, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data, c_data,
+                kept_out);
+}
+
 hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* tiles,
                               const int32_t* perm, const Entry* entries, const Band* bands,
                               const double* a_pack, const double* b_pack, uint32_t a_zero,
@@ -544,6 +710,14 @@ hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* ti
                        eps2, old_row_p, old_col, old_off, old_data, c_data, kept);
   else if (variant == kF64Ksplit)
     hipLaunchKernelGGL(numeric_f64_ksplit_kernel<false>, dim3((unsigned)n_tiles), dim3(256), 0, s,
+                       tiles, perm, entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta,
+                       eps2, old_row_p, old_col, old_off, old_data, c_data, kept);
+  else if (variant == kF64Half && whole)
+    hipLaunchKernelGGL(numeric_f64_half_kernel<true>, dim3((unsigned)n_tiles), dim3(256), 0, s,
+                       tiles, perm, entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta,
+                       eps2, old_row_p, old_col, old_off, old_data, c_data, kept);
+  else if (variant == kF64Half)
+    hipLaunchKernelGGL(numeric_f64_half_kernel<false>, dim3((unsigned)n_tiles), dim3(256), 0, s,
                        tiles, perm, entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta,
                        eps2, old_row_p, old_col, old_off, old_data, c_data, kept);
   else

@@ -111,12 +111,13 @@ static hipError_t record_event(lamb_hip_ctx* ctx, int k) {
   return hipEventRecord(ctx->ev[k], ctx->stream);
 }
 
-// FP64 kernel variant forced by LAMB_HIP_F64_KERNEL (quadrant | ksplit), or
+// FP64 kernel variant forced by LAMB_HIP_F64_KERNEL (quadrant | half | ksplit), or
 // -1: chosen by the autotuner
 static int f64_forced() {
   const char* v = getenv("LAMB_HIP_F64_KERNEL");
   if (v && strcmp(v, "quadrant") == 0) return kF64Quadrant;
   if (v && strcmp(v, "ksplit") == 0) return kF64Ksplit;
+  if (v && strcmp(v, "half") == 0) return kF64Half;
   return -1;
 }
 
@@ -291,8 +292,10 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
   };
   int variant = 0;
   if (!split) {
-    // Autotuner: the FP64 variants trade occupancy (quadrant) against work
-    // per K step (ksplit); which wins depends on how dense the tiles are. A
+    // Autotuner: the FP64 variants trade the number of waves that walk each
+    // K step (quadrant: up to 4, half: 2, ksplit: 1) against accumulator
+    // registers and occupancy; which wins depends on how dense the tiles
+    // are. A
     // workload class is (form and whether every entry is whole, largest
     // block, flops per tile, C blocks per tile) in half-octave buckets; the
     // first multiply of a class times every variant on its own tiles (the
@@ -308,10 +311,10 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
       if (it != ctx->tune.end()) {
         variant = it->second;
       } else if (ctx->size_mode == kSizesReplay) {
-        variant = kF64Quadrant;  // no timing while capturing (the recorded run tuned)
+        variant = kF64Half;  // no timing while capturing (the recorded run tuned)
       } else {
         float best = 0.f;
-        for (int v : {kF64Quadrant, kF64Ksplit}) {
+        for (int v : {kF64Quadrant, kF64Half, kF64Ksplit}) {
           HIPCHK(ctx, record_event(ctx, 5), "event");
           HIPCHK(ctx, launch_f64(v), "numeric kernel (tuning)");
           HIPCHK(ctx, record_event(ctx, 6), "event");

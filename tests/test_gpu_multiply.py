@@ -187,7 +187,7 @@ def test_autotuner_is_deterministic_and_variants_agree(monkeypatch):
         print(f"autotuner chose variant {v1} ({n1} class)", flush=True)
         assert (v1, n1) == (v2, n2) and n1 == 1
         assert np.array_equal(got1.data, got2.data)
-        for name in ("quadrant", "ksplit"):
+        for name in ("quadrant", "half", "ksplit"):
             monkeypatch.setenv("LAMB_HIP_F64_KERNEL", name)
             got, _ = gpu_multiply(fresh, atoms, a, b, eps=1e-7)
             same, err = compare(got, ref)
@@ -196,7 +196,7 @@ def test_autotuner_is_deterministic_and_variants_agree(monkeypatch):
         fresh.close()
 
 
-@pytest.mark.parametrize("variant", ["quadrant", "ksplit"])
+@pytest.mark.parametrize("variant", ["quadrant", "half", "ksplit"])
 def test_forced_variants_whole_and_partial_products(ctx, monkeypatch, variant):
     """Each FP64 variant on a product whose entries are all whole (unmasked
     loads) and on one where the prefilter cuts entries (per-lane masks)."""
