@@ -23,7 +23,7 @@ PEAK_HBM_GBS = 8000.0
 # workload (numeric_f64_half_kernel): rocprofv3 FETCH_SIZE x 2.000 +
 # WRITE_SIZE x 1.000, the factors read off a device copy of known size in the
 # same run (profiles/numeric_f64_half_kernel_water6144.md).
-PROFILED_TRAFFIC_BYTES = 106.84e9
+PROFILED_TRAFFIC_BYTES = 106.38e9
 # FP64 numeric kernel of each autotuner variant (lamb_hip_tuning_info)
 F64_KERNELS = {0: "numeric_f64_kernel", 1: "numeric_f64_ksplit_kernel",
                2: "numeric_f64_half_kernel"}

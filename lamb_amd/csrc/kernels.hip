@@ -437,21 +437,22 @@ __global__ __launch_bounds__(256, 2) void numeric_f64_ksplit_kernel(
     {                                                                              \
       uint32_t bm = 0;                                                             \
       int st = 0;                                                                  \
-      for (;;) {                                                                   \
-        while (live == 0 && cbase + 64 < nsteps) {                                 \
-          cbase += 64;                                                             \
-          const int sl = cbase + lane;                                             \
-          chunk_bm = tile_blocks(sl < nsteps ? step_g[sl] : 0u);                   \
-          live = __ballot(chunk_bm != 0u);                                         \
-        }                                                                          \
-        if (live == 0) break;                                                      \
+      while (live == 0 && cbase + 64 < nsteps) {                                   \
+        cbase += 64;                                                               \
+        const int sl = cbase + lane;                                               \
+        chunk_bm = tile_blocks(sl < nsteps ? step_g[sl] : 0u);                     \
+        /* this wave's live steps: those whose rank is wave mod 4 */               \
+        const uint64_t any = __ballot(chunk_bm != 0u);                             \
+        const uint32_t rank = seen + __builtin_amdgcn_mbcnt_hi(                    \
+            (uint32_t)(any >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)any, 0u));  \
+        live = __ballot(chunk_bm != 0u && (rank & 3u) == (uint32_t)wave);          \
+        seen += (uint32_t)__builtin_popcountll(any);                               \
+      }                                                                            \
+      if (live) {                                                                  \
         const int l = __builtin_ctzll(live);                                       \
         live &= live - 1;                                                          \
-        if ((seen++ & 3u) == (uint32_t)wave) {                                     \
-          st = cbase + l;                                                          \
-          bm = __builtin_amdgcn_readlane(chunk_bm, l);                             \
-          break;                                                                   \
-        }                                                                          \
+        st = cbase + l;                                                            \
+        bm = __builtin_amdgcn_readlane(chunk_bm, l);                               \
       }                                                                            \
       blkmask[S] = bm;                                                             \
       const int Q = 4 * st + kc;                                                   \
@@ -604,21 +605,22 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
     {                                                                              \
       uint32_t bm = 0;                                                             \
       int st = 0;                                                                  \
-      for (;;) {                                                                   \
-        while (live == 0 && cbase + 64 < nsteps) {                                 \
-          cbase += 64;                                                             \
-          const int sl = cbase + lane;                                             \
-          chunk_bm = tile_blocks(sl < nsteps ? step_g[sl] : 0u) & half_bits;       \
-          live = __ballot(chunk_bm != 0u);                                         \
-        }                                                                          \
-        if (live == 0) break;                                                      \
+      while (live == 0 && cbase + 64 < nsteps) {                                   \
+        cbase += 64;                                                               \
+        const int sl = cbase + lane;                                               \
+        chunk_bm = tile_blocks(sl < nsteps ? step_g[sl] : 0u) & half_bits;         \
+        /* this wave's live steps: those of rank parity par in the half */         \
+        const uint64_t any = __ballot(chunk_bm != 0u);                             \
+        const uint32_t rank = seen + __builtin_amdgcn_mbcnt_hi(                    \
+            (uint32_t)(any >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)any, 0u));  \
+        live = __ballot(chunk_bm != 0u && (rank & 1u) == par);                     \
+        seen += (uint32_t)__builtin_popcountll(any);                               \
+      }                                                                            \
+      if (live) {                                                                  \
         const int l = __builtin_ctzll(live);                                       \
         live &= live - 1;                                                          \
-        if ((seen++ & 1u) == par) {                                                \
-          st = cbase + l;                                                          \
-          bm = __builtin_amdgcn_readlane(chunk_bm, l);                             \
-          break;                                                                   \
-        }                                                                          \
+        st = cbase + l;                                                            \
+        bm = __builtin_amdgcn_readlane(chunk_bm, l);                               \
       }                                                                            \
       blkmask[S] = bm;                                                             \
       const int Q = 4 * st + kc;                                                   \
