@@ -112,7 +112,10 @@ int32_t lamb_hip_goods_unpack(lamb_hip_mat* m, int32_t sr, int32_t sc, const int
                      (long long)n_index_words);
   const int64_t n_rows = (int64_t)d->subset_atoms[sr - 1].size();
   const int64_t n_blks = index[7];
-  if (index[5] != n_rows || n_blks < 0 || index[9] != n_blks || index[11] != n_blks)
+  // every block takes two extents words, so n_blks <= n_index_words bounds the
+  // header's count before any size is computed from it
+  if (index[5] != n_rows || n_blks < 0 || n_blks > n_index_words || index[9] != n_blks ||
+      index[11] != n_blks)
     return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_WRONG_ARGS,
                      "goods_unpack: header holds %lld rows and %lld / %lld / %lld blocks, "
                      "subset %d has %lld rows",
