@@ -380,6 +380,16 @@ int32_t lamb_hip_sector_upload(lamb_hip_mat* m, int32_t sr, int32_t sc, int64_t 
     return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_WRONG_ARGS,
                      "sector_upload: row_p[0]=%lld row_p[n_rows]=%lld, expected 0 and %lld",
                      (long long)row_p[0], (long long)row_p[n_rows], (long long)n_blks);
+  // row r's blocks are row_p[r]+1 .. row_p[r+1] (src/lamb_index_methods.F90:91-92):
+  // the offsets must be non-decreasing inside [0, n_blks], or the commit
+  // would read col_i / extents past their ends
+  for (int64_t r = 0; r < n_rows; ++r)
+    if (row_p[r] < 0 || row_p[r] > row_p[r + 1] || row_p[r + 1] > n_blks)
+      return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_WRONG_ARGS,
+                       "sector_upload: row_p[%lld..%lld] = %lld, %lld is not a non-decreasing "
+                       "pair inside [0, %lld]",
+                       (long long)r, (long long)r + 1, (long long)row_p[r],
+                       (long long)row_p[r + 1], (long long)n_blks);
   StagedSector st;
   st.sr = sr;
   st.sc = sc;
@@ -420,6 +430,27 @@ int32_t lamb_hip_matrix_commit(lamb_hip_mat* m) {
   std::vector<Blk> blks;
   int64_t total = 0;
   std::vector<int64_t> base;
+  // extents [lo, hi] per data store, to refuse two blocks sharing values
+  struct Ext {
+    int64_t lo, hi;
+    int32_t gr, gc;
+  };
+  std::vector<Ext> store_ext;
+  auto check_overlap = [&](int64_t store_no) -> int32_t {
+    std::sort(store_ext.begin(), store_ext.end(),
+              [](const Ext& a, const Ext& b) { return a.lo < b.lo; });
+    for (size_t t = 1; t < store_ext.size(); ++t)
+      if (store_ext[t].lo <= store_ext[t - 1].hi)
+        return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_WRONG_ARGS,
+                         "commit: blocks (%d,%d) and (%d,%d) overlap in data store %lld "
+                         "(extents [%lld,%lld] and [%lld,%lld])",
+                         store_ext[t - 1].gr + 1, store_ext[t - 1].gc + 1, store_ext[t].gr + 1,
+                         store_ext[t].gc + 1, (long long)store_no + 1,
+                         (long long)store_ext[t - 1].lo, (long long)store_ext[t - 1].hi,
+                         (long long)store_ext[t].lo, (long long)store_ext[t].hi);
+    store_ext.clear();
+    return LAMB_HIP_OK;
+  };
   auto atom_of = [&](int32_t s, int64_t local1, int32_t& g) -> bool {
     if (s < 1 || s > d->n_subsets) return false;
     const auto& v = d->subset_atoms[s - 1];
@@ -437,8 +468,10 @@ int32_t lamb_hip_matrix_commit(lamb_hip_mat* m) {
                        gr + 1, gc + 1, (long long)lo, (long long)hi, (long long)len,
                        (long long)store_len);
     blks.push_back({d->perm[gr], d->perm[gc], store_base + lo - 1});
+    if (len > 0) store_ext.push_back({lo, hi, gr, gc});
     return LAMB_HIP_OK;
   };
+  int64_t store_no = 0;
   for (const auto& st : m->staged_sectors) {
     base.push_back(total);
     const int64_t sb = total;
@@ -457,6 +490,7 @@ int32_t lamb_hip_matrix_commit(lamb_hip_mat* m) {
         if (rc) return rc;
       }
     }
+    if (int32_t rc = check_overlap(store_no++)) return rc;
     total += (int64_t)st.data.size();
   }
   for (const auto& st : m->staged_lists) {
@@ -471,6 +505,7 @@ int32_t lamb_hip_matrix_commit(lamb_hip_mat* m) {
       int32_t rc = add(gr, gc, st.ext[2 * b], st.ext[2 * b + 1], (int64_t)st.data.size(), total);
       if (rc) return rc;
     }
+    if (int32_t rc = check_overlap(store_no++)) return rc;
     total += (int64_t)st.data.size();
   }
   std::sort(blks.begin(), blks.end(), [](const Blk& a, const Blk& b) {
