@@ -19,12 +19,12 @@ import numpy as np
 METRIC = "block-SpGEMM GFLOP/s & HBM GB/s: FP64 plain vs BF16x3 batched at ≤1e-9 err"
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per launch of the FP64 numeric kernel the autotuner picks on this
-# workload (numeric_f64_half_kernel): rocprofv3 FETCH_SIZE x 2.000 +
-# WRITE_SIZE x 1.000, the factors read off a device copy of known size in the
-# same run (profiles/numeric_f64_half_kernel_water6144.md).
-PROFILED_TRAFFIC_BYTES = 106.38e9
-# FP64 numeric kernel of each autotuner variant (lamb_hip_tuning_info)
+# HBM bytes per launch of an FP64 numeric kernel on this workload, by kernel
+# name: rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scale factors from
+# a device copy of known size in the same run), from the committed profile
+# named after the kernel. A kernel without a profile reports null.
+PROFILED_TRAFFIC_BYTES = {}
+# FP64 numeric kernel of each variant (lamb_hip_tuning_info)
 F64_KERNELS = {0: "numeric_f64_kernel", 1: "numeric_f64_ksplit_kernel",
                2: "numeric_f64_half_kernel"}
 
@@ -408,6 +408,7 @@ def main():
     ms_num = float(np.mean([s["ms_numeric"] for s in sts]))
     achieved_tf = flops / (ms_num * 1e-3) / 1e12
     variant, _ = D.tuning_info(ctx)
+    kname = F64_KERNELS.get(variant, str(variant))
     result = {
         "metric": METRIC,
         "value": flops / (ms_step * 1e-3) / 1e9,
@@ -442,12 +443,12 @@ def main():
         },
         "roofline": {
             "bound": "mfma",
-            "kernel": F64_KERNELS.get(variant, str(variant)),
+            "kernel": kname,
             "achieved": achieved_tf,
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tf / PEAK_FP64_TFLOPS,
-            "traffic": PROFILED_TRAFFIC_BYTES,
+            "traffic": PROFILED_TRAFFIC_BYTES.get(kname),
             "algorithmic_bytes": bytes_alg,
         },
     }

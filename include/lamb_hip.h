@@ -173,12 +173,19 @@ int32_t lamb_hip_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A, const lamb_h
 /* tr(A): sum of the diagonal elements of the diagonal blocks (FP64). */
 int32_t lamb_hip_trace(const lamb_hip_mat* A, double* trace);
 
-/* Autotuner state (no counterpart in the reference): the FP64 numeric
- * kernel variant of the context's last multiply (0 quadrant: per-wave
- * 32 x 32 quadrants; 1 ksplit: whole-tile waves splitting the K steps) and
- * the number of workload classes tuned so far. The first multiply of a class
- * (form, largest block, flops and C blocks per tile) times every variant and
- * keeps the fastest; LAMB_HIP_F64_KERNEL=quadrant|ksplit forces one. */
+/* Kernel choice and autotuner state (no counterpart in the reference).
+ * last_variant: the FP64 numeric kernel of the context's last multiply
+ * (0 quadrant: per-wave 32 x 32 quadrants; 1 ksplit: whole-tile waves
+ * splitting the K steps four ways; 2 half: per-wave 64 x 32 halves, two waves
+ * splitting each half's K steps). The variant is a function of the product
+ * (K-split for whole products of at least 1e7 flop per tile, else half), so
+ * a product's bits do not depend on timings; LAMB_HIP_F64_KERNEL=
+ * quadrant|half|ksplit forces one (the variants add K products in different
+ * orders: results then differ at rounding level). n_classes: workload classes
+ * tuned so far. The autotuner times the tile orders (row bands per block of
+ * the order in which workgroups take tiles: 4, 8, 16) on the first product of
+ * a class and keeps the fastest; the order only maps tiles to workgroups, so
+ * every candidate writes the same bits. */
 int32_t lamb_hip_tuning_info(const lamb_hip_ctx* ctx, int32_t* last_variant,
                              int32_t* n_classes);
 

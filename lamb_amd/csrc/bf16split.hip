@@ -349,9 +349,8 @@ __global__ __launch_bounds__(256, 2) void numeric_bf16_kernel(
   }
   flush();
   __syncthreads();
-  const double* Ct = ctile;
-  tile_epilogue(Ct, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data, c_data,
-                kept_out);
+  tile_epilogue<256>(ctile, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data,
+                     c_data, kept_out);
 }
 
 // ---------------------------------------------------------------------------

@@ -77,9 +77,10 @@ struct lamb_hip_ctx {
   lamb::DevBuf d_aexp, d_bexp;  // BF16 split: exponent per A row / B column
   lamb::DevBuf d_aslice, d_bslice;  // BF16 split: slice packs
   void* sym = nullptr;  // lamb::SymState*
-  // autotuner: numeric kernel variant per workload class (multiply.hip)
+  // autotuner: tile-order row-block size per workload class (multiply.hip)
   std::map<std::array<int64_t, 4>, int32_t> tune;
-  int32_t last_variant = -1;
+  int32_t last_variant = -1;   // FP64 variant of the last multiply
+  int32_t last_rowblock = -1;  // tile-order row bands per block of the last multiply
   // Sizes the host reads back from the device (graph.hip). kSizesLive: read
   // them (stream sync); kSizesRecord: read and append to size_log;
   // kSizesReplay (during a graph capture): take them from size_log in order

@@ -345,8 +345,8 @@ __global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
         Ct[row * kPitch + col] = acc[2 * h + q][r];
       }
   __syncthreads();
-  tile_epilogue(Ct, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data, c_data,
-                kept_out);
+  tile_epilogue<256>(Ct, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data,
+                     c_data, kept_out);
 }
 
 // ---------------------------------------------------------------------------
@@ -527,8 +527,8 @@ __global__ __launch_bounds__(256, 2) void numeric_f64_ksplit_kernel(
     }
     __syncthreads();
   }
-  tile_epilogue(Ct, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data, c_data,
-                kept_out);
+  tile_epilogue<256>(Ct, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data,
+                     c_data, kept_out);
 }
 
 // ---------------------------------------------------------------------------
@@ -694,8 +694,8 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
     }
     __syncthreads();
   }
-  tile_epilogue(Ct, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data, c_data,
-                kept_out);
+  tile_epilogue<256>(Ct, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data,
+                     c_data, kept_out);
 }
 
 hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* tiles,

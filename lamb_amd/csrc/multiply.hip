@@ -111,8 +111,8 @@ static hipError_t record_event(lamb_hip_ctx* ctx, int k) {
   return hipEventRecord(ctx->ev[k], ctx->stream);
 }
 
-// FP64 kernel variant forced by LAMB_HIP_F64_KERNEL (quadrant | half | ksplit), or
-// -1: chosen by the autotuner
+// FP64 kernel variant forced by LAMB_HIP_F64_KERNEL (quadrant | half | ksplit),
+// or -1
 static int f64_forced() {
   const char* v = getenv("LAMB_HIP_F64_KERNEL");
   if (v && strcmp(v, "quadrant") == 0) return kF64Quadrant;
@@ -120,6 +120,23 @@ static int f64_forced() {
   if (v && strcmp(v, "half") == 0) return kF64Half;
   return -1;
 }
+
+// The FP64 variant of a product: a function of the workload only, so that a
+// product's bits do not depend on timings (the variants add K products in
+// different orders). Tiles of a dense product (flops per tile at or above
+// 1e7, e.g. the iterates of a purification chain, about 1.8e8) take the
+// K-split kernel when no entry needs per-lane masks; every other product the
+// half kernel. Measured: 6,144-atom water box (9.0e5 flop per tile) half
+// 16.0 ms, quadrant 17.8, K-split 20.3; a dense 16 A product of 3,072 atoms
+// K-split 251 ms, half 279, quadrant 344.
+constexpr double kDenseTileFlops = 1.0e7;
+static int f64_rule(double flops_per_tile, bool whole) {
+  return (whole && flops_per_tile >= kDenseTileFlops) ? kF64Ksplit : kF64Half;
+}
+
+// Tile-order block sizes (row bands per block) the autotuner times. The order
+// only maps tiles to workgroups, so every candidate writes the same bits.
+static const int kRowBlocks[] = {4, 8, 16};
 
 // half-octave bucket of a positive size
 static int64_t bucket(double x) { return (int64_t)std::floor(2.0 * std::log2(std::max(x, 1.0))); }
@@ -261,16 +278,37 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
   if (S.r.n_entries > INT32_MAX || S.r.n_tiles > INT32_MAX)
     return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_OVERFLOW_ERROR,
                      "multiply: symbolic sizes exceed 32-bit indices");
-  {
-    // tiles run in blocks of rb row bands (tuning knob, default 8)
-    const char* env = getenv("LAMB_HIP_TILE_ROWBLOCK");
-    const int rb = env ? atoi(env) : 8;
-    HIPCHK(ctx, build_tile_order(ctx, S.W, D, S.r, std::max(1, std::min(rb, 32))), "tile order");
+  // FP64 variant (deterministic) and tile order. The autotuner times the
+  // candidate tile orders on the first product of a workload class (form,
+  // variant, largest block, half-octave buckets of flops per tile and of C
+  // blocks per tile) and keeps the fastest for the context's lifetime; a
+  // graph capture with no tuned class uses 8 row bands per block.
+  const int32_t n_tiles = (int32_t)S.r.n_tiles;
+  const double nt = std::max<double>(1.0, (double)n_tiles);
+  const bool whole = S.r.n_partial == 0;
+  int variant = 0;
+  if (!split) {
+    const int forced = f64_forced();
+    variant = forced >= 0 ? forced : f64_rule(S.r.flops / nt, whole);
+    ctx->last_variant = variant;
   }
+  const std::array<int64_t, 4> key = {
+      (int64_t)flags | ((int64_t)whole << 32) | ((int64_t)variant << 40), d->max_blk,
+      bucket(S.r.flops / nt), bucket((double)S.r.n_struct / nt)};
+  int rb = 8;
+  bool tune_order = false;
+  if (const char* env = getenv("LAMB_HIP_TILE_ROWBLOCK")) {
+    rb = std::max(1, std::min(atoi(env), 32));
+  } else if (auto it = ctx->tune.find(key); it != ctx->tune.end()) {
+    rb = it->second;
+  } else if (ctx->size_mode != kSizesReplay && n_tiles > 0) {
+    tune_order = true;
+    rb = kRowBlocks[0];
+  }
+  HIPCHK(ctx, build_tile_order(ctx, S.W, D, S.r, rb), "tile order");
   HIPCHK(ctx, record_event(ctx, 3), "event");
 
   // ---- numeric ----
-  const int32_t n_tiles = (int32_t)S.r.n_tiles;
   HIPCHK(ctx, ctx->d_kept.ensure(sizeof(uint16_t) * kBandAtoms * (size_t)std::max(n_tiles, 1)),
          "kept");
   // the new C goes into C's spare buffers, swapped in at the end (a graph
@@ -279,71 +317,50 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
   C->s_data = DevBuf();
   HIPCHK(ctx, newdata.ensure(sizeof(double) * (size_t)std::max<int64_t>(S.r.c_size, 1)),
          "C pool");
-  auto launch_f64 = [&](int variant) {
-    return launch_numeric_f64(s, n_tiles, (const TileDesc*)S.r.tiles.p,
-                              (const int32_t*)S.r.perm.p, (const Entry*)S.r.entries.p, D.bands,
-                              (const double*)ctx->d_apack.p, (const double*)ctx->d_bpack.p,
-                              a_zero, b_zero, alpha, beta, eps2,
-                              use_old ? MC.row_p : nullptr, use_old ? MC.col : nullptr,
-                              use_old ? MC.off : nullptr,
-                              use_old ? (const double*)C->d_data.p : nullptr,
-                              (double*)newdata.p, (uint16_t*)ctx->d_kept.p, variant,
-                              S.r.n_partial == 0);
+  auto launch_numeric = [&]() -> hipError_t {
+    if (!split)
+      return launch_numeric_f64(s, n_tiles, (const TileDesc*)S.r.tiles.p,
+                                (const int32_t*)S.r.perm.p, (const Entry*)S.r.entries.p, D.bands,
+                                (const double*)ctx->d_apack.p, (const double*)ctx->d_bpack.p,
+                                a_zero, b_zero, alpha, beta, eps2,
+                                use_old ? MC.row_p : nullptr, use_old ? MC.col : nullptr,
+                                use_old ? MC.off : nullptr,
+                                use_old ? (const double*)C->d_data.p : nullptr,
+                                (double*)newdata.p, (uint16_t*)ctx->d_kept.p, variant, whole);
+    return launch_numeric_bf16(s, slices, n_tiles, (const TileDesc*)S.r.tiles.p,
+                               (const int32_t*)S.r.perm.p, (const Entry*)S.r.entries.p, D.bands,
+                               ctx->d_aslice.p, ctx->d_bslice.p, a_zero, b_zero, alpha, beta,
+                               eps2, use_old ? MC.row_p : nullptr, use_old ? MC.col : nullptr,
+                               use_old ? MC.off : nullptr,
+                               use_old ? (const double*)C->d_data.p : nullptr,
+                               (double*)newdata.p, (uint16_t*)ctx->d_kept.p);
   };
-  int variant = 0;
-  if (!split) {
-    // Autotuner: the FP64 variants trade the number of waves that walk each
-    // K step (quadrant: up to 4, half: 2, ksplit: 1) against accumulator
-    // registers and occupancy; which wins depends on how dense the tiles
-    // are. A
-    // workload class is (form and whether every entry is whole, largest
-    // block, flops per tile, C blocks per tile) in half-octave buckets; the
-    // first multiply of a class times every variant on its own tiles (the
-    // launches only rewrite this call's C pool) and the winner is kept for
-    // the context's lifetime.
-    variant = f64_forced();
-    if (variant < 0) {
-      const double nt = std::max<double>(1.0, (double)n_tiles);
-      const int64_t form = (int64_t)flags | ((int64_t)(S.r.n_partial == 0) << 32);
-      const std::array<int64_t, 4> key = {form, d->max_blk, bucket(S.r.flops / nt),
-                                           bucket((double)S.r.n_struct / nt)};
-      auto it = ctx->tune.find(key);
-      if (it != ctx->tune.end()) {
-        variant = it->second;
-      } else if (ctx->size_mode == kSizesReplay) {
-        variant = kF64Half;  // no timing while capturing (the recorded run tuned)
-      } else {
-        float best = 0.f;
-        for (int v : {kF64Quadrant, kF64Half, kF64Ksplit}) {
-          HIPCHK(ctx, record_event(ctx, 5), "event");
-          HIPCHK(ctx, launch_f64(v), "numeric kernel (tuning)");
-          HIPCHK(ctx, record_event(ctx, 6), "event");
-          HIPCHK(ctx, hipEventSynchronize(ctx->ev[6]), "tuning sync");
-          float ms = 0.f;
-          HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[5], ctx->ev[6]), "event time");
-          if (v == kF64Quadrant || ms < best) {
-            best = ms;
-            variant = v;
-          }
-        }
-        ctx->tune[key] = variant;
+  if (tune_order) {
+    // every candidate writes the same C pool and kept masks; the last
+    // launch's result is the call's, its time the call's numeric time
+    float best = 0.f;
+    int best_rb = rb;
+    for (size_t c = 0; c < sizeof(kRowBlocks) / sizeof(kRowBlocks[0]); ++c) {
+      if (c > 0) HIPCHK(ctx, build_tile_order(ctx, S.W, D, S.r, kRowBlocks[c]), "tile order");
+      HIPCHK(ctx, record_event(ctx, 0), "event");
+      HIPCHK(ctx, launch_numeric(), "numeric kernel (tuning)");
+      HIPCHK(ctx, record_event(ctx, 1), "event");
+      HIPCHK(ctx, hipEventSynchronize(ctx->ev[1]), "tuning sync");
+      float ms = 0.f;
+      HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]), "event time");
+      if (c == 0 || ms < best) {
+        best = ms;
+        best_rb = kRowBlocks[c];
       }
     }
-    ctx->last_variant = variant;
+    ctx->tune[key] = best_rb;
+    rb = best_rb;
+  } else {
+    HIPCHK(ctx, record_event(ctx, 0), "event");
+    HIPCHK(ctx, launch_numeric(), split ? "numeric kernel (BF16 split)" : "numeric kernel");
+    HIPCHK(ctx, record_event(ctx, 1), "event");
   }
-  HIPCHK(ctx, record_event(ctx, 0), "event");
-  if (!split)
-    HIPCHK(ctx, launch_f64(variant), "numeric kernel");
-  else
-    HIPCHK(ctx, launch_numeric_bf16(s, slices, n_tiles, (const TileDesc*)S.r.tiles.p,
-                                    (const int32_t*)S.r.perm.p, (const Entry*)S.r.entries.p,
-                                    D.bands, ctx->d_aslice.p, ctx->d_bslice.p, a_zero, b_zero,
-                                    alpha, beta, eps2, use_old ? MC.row_p : nullptr,
-                                    use_old ? MC.col : nullptr, use_old ? MC.off : nullptr,
-                                    use_old ? (const double*)C->d_data.p : nullptr,
-                                    (double*)newdata.p, (uint16_t*)ctx->d_kept.p),
-           "numeric kernel (BF16 split)");
-  HIPCHK(ctx, record_event(ctx, 1), "event");
+  ctx->last_rowblock = rb;
 
   // ---- C index ----
   CIndex ci;

@@ -17,6 +17,7 @@ struct TileSmem {
   int32_t wave_cnt[4];
   int64_t wave_sz[4];
   uint32_t keptw[kBandAtoms];
+  uint8_t r8_r[kBandAtoms], r8_c[kBandAtoms];  // 8-row / 8-col regions per atom
 };
 
 struct TileInfo {
@@ -71,6 +72,13 @@ __device__ __forceinline__ TileInfo tile_setup(const TileDesc* __restrict__ tile
     if (tid < t.natJ && sm.rs_c[tid + 1] > sm.rs_c[tid])
       for (int q = sm.rs_c[tid] / 4; q <= (sm.rs_c[tid + 1] - 1) / 4; ++q) g |= 1u << q;
     sm.pg_c[tid] = (uint16_t)g;
+    uint32_t r8 = 0, c8 = 0;
+    if (tid < t.natI && sm.rs_r[tid + 1] > sm.rs_r[tid])
+      for (int q = sm.rs_r[tid] / 8; q <= (sm.rs_r[tid + 1] - 1) / 8; ++q) r8 |= 1u << q;
+    if (tid < t.natJ && sm.rs_c[tid + 1] > sm.rs_c[tid])
+      for (int q = sm.rs_c[tid] / 8; q <= (sm.rs_c[tid + 1] - 1) / 8; ++q) c8 |= 1u << q;
+    sm.r8_r[tid] = (uint8_t)r8;
+    sm.r8_c[tid] = (uint8_t)c8;
   }
   __syncthreads();
   return t;
@@ -119,55 +127,73 @@ __device__ __forceinline__ uint32_t quadrant_blocks(uint32_t g, uint32_t gsh_r, 
   return (r0 & c0) | ((r0 & c1) << 1) | ((r1 & c0) << 2) | ((r1 & c1) << 3);
 }
 
-// Epilogue. Ct holds the tile's 64 x 64 accumulated products (row pitch
-// kPitch) and is visible to all threads. Each structural block (i,j) of the
-// tile becomes alpha * (A*B)_ij + beta * Cold_ij (Cold found by binary search
-// in its CSR row), is written to the C pool at out_base + its offset
-// (blocks back to back in row-major order of atom pairs, each column-major),
-// and is kept iff its sum of squares (sequential FMA order within a lane,
-// then a wave sum) is >= eps2. kept_out[tile*16 + i] gets the kept columns
-// of band atom i.
+// Epilogue for a workgroup of NT threads (NT / 64 waves). Ct holds the tile's
+// 64 x 64 accumulated products (row pitch kPitch) and is visible to all
+// threads. Each structural block (i,j) of the tile becomes
+// alpha * (A*B)_ij + beta * Cold_ij (Cold found by binary search in its CSR
+// row; each operation rounded on its own, no contraction), is written to the
+// C pool at out_base + its offset (blocks back to back in row-major order of
+// atom pairs, each column-major) and back into Ct. Its sum of squares is then
+// taken by one thread as one sequential FMA chain over the block's values in
+// column-major order, s = fma(x, x, s) from s = +0, the same order as the
+// prefilter's block norms; the block is kept iff that sum is >= eps2.
+// kept_out[tile*16 + i] gets the kept columns of band atom i.
+template <int NT>
 __device__ __forceinline__ void tile_epilogue(
-    const double* Ct, const TileInfo& t, TileSmem& sm, double alpha, double beta, double eps2,
+    double* Ct, const TileInfo& t, TileSmem& sm, double alpha, double beta, double eps2,
     const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
     const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
     double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
+#pragma clang fp contract(off)
+  static_assert(NT % 64 == 0 && NT >= 64 && NT <= 256, "epilogue threads");
+  constexpr int kWaves = NT / 64;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  // structural pairs (row-major over band positions) and their offsets
-  const int ip = tid >> 4, jp = tid & 15;
-  const bool present = ip < t.natI && jp < t.natJ && ((t.tp->pattern[ip] >> jp) & 1);
-  const int64_t sz =
-      present ? (int64_t)(sm.rs_r[ip + 1] - sm.rs_r[ip]) * (sm.rs_c[jp + 1] - sm.rs_c[jp]) : 0;
-  int cnt = present ? 1 : 0;
-  int64_t csz = sz;
-  for (int o = 1; o < 64; o <<= 1) {
-    const int c2 = __shfl_up(cnt, o, 64);
-    const int64_t s2 = __shfl_up(csz, o, 64);
-    if (lane >= o) {
-      cnt += c2;
-      csz += s2;
-    }
-  }
-  if (lane == 63) {
-    sm.wave_cnt[wave] = cnt;
-    sm.wave_sz[wave] = csz;
-  }
-  __syncthreads();
+  // structural pairs (row-major over band positions: id = 16 ip + jp) and
+  // their offsets, NT ids per pass
   int cbase = 0;
   int64_t sbase = 0;
-  for (int w = 0; w < wave; ++w) {
-    cbase += sm.wave_cnt[w];
-    sbase += sm.wave_sz[w];
+  for (int id0 = 0; id0 < kBandAtoms * kBandAtoms; id0 += NT) {
+    const int id = id0 + tid;
+    const int ip = id >> 4, jp = id & 15;
+    const bool present = ip < t.natI && jp < t.natJ && ((t.tp->pattern[ip] >> jp) & 1);
+    const int64_t sz =
+        present ? (int64_t)(sm.rs_r[ip + 1] - sm.rs_r[ip]) * (sm.rs_c[jp + 1] - sm.rs_c[jp]) : 0;
+    int cnt = present ? 1 : 0;
+    int64_t csz = sz;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int c2 = __shfl_up(cnt, o, 64);
+      const int64_t s2 = __shfl_up(csz, o, 64);
+      if (lane >= o) {
+        cnt += c2;
+        csz += s2;
+      }
+    }
+    if (lane == 63) {
+      sm.wave_cnt[wave] = cnt;
+      sm.wave_sz[wave] = csz;
+    }
+    __syncthreads();
+    int cw = cbase;
+    int64_t sw = sbase;
+    for (int w = 0; w < wave; ++w) {
+      cw += sm.wave_cnt[w];
+      sw += sm.wave_sz[w];
+    }
+    if (present) {
+      sm.pair_id[cw + cnt - 1] = id;
+      sm.pair_off[cw + cnt - 1] = sw + csz - sz;
+    }
+    for (int w = 0; w < kWaves; ++w) {
+      cbase += sm.wave_cnt[w];
+      sbase += sm.wave_sz[w];
+    }
+    __syncthreads();
   }
-  const int npairs = sm.wave_cnt[0] + sm.wave_cnt[1] + sm.wave_cnt[2] + sm.wave_cnt[3];
-  if (present) {
-    sm.pair_id[cbase + cnt - 1] = tid;
-    sm.pair_off[cbase + cnt - 1] = sbase + csz - sz;
-  }
-  __syncthreads();
+  const int npairs = cbase;
 
-  for (int pi = wave; pi < npairs; pi += 4) {
+  // values: one wave per block, lanes over its elements
+  for (int pi = wave; pi < npairs; pi += kWaves) {
     const int id = sm.pair_id[pi];
     const int pr = id >> 4, pc = id & 15;
     const int m = sm.rs_r[pr + 1] - sm.rs_r[pr];
@@ -191,16 +217,55 @@ __device__ __forceinline__ void tile_epilogue(
       }
       if (found >= 0) old = old_data + old_off[found];
     }
-    double part = 0.0;
+    // element e = r + m * cc of the column-major block, stepped by 64
+    // without a division per element
+    const int q64 = 64 / m, r64 = 64 - q64 * m;
+    int cc = lane / m, r = lane - (lane / m) * m;
     for (int e = lane; e < len; e += 64) {
-      const int cc = e / m, r = e - cc * m;
-      double v = alpha * Ct[(sm.rs_r[pr] + r) * kPitch + sm.rs_c[pc] + cc];
-      if (old) v = v + beta * old[e];
+      double* x = &Ct[(sm.rs_r[pr] + r) * kPitch + sm.rs_c[pc] + cc];
+      double v = alpha * *x;
+      if (old) {
+        const double bo = beta * old[e];
+        v = v + bo;
+      }
       c_data[base + e] = v;
-      part = __builtin_fma(v, v, part);
+      *x = v;
+      r += r64;
+      cc += q64;
+      if (r >= m) {
+        r -= m;
+        cc += 1;
+      }
     }
-    const double nsq = wave_sum(part);
-    if (lane == 0 && nsq >= eps2) atomicOr(&sm.keptw[pr], 1u << pc);
+  }
+  __syncthreads();
+  // norms: one thread per block, one FMA chain over its values in
+  // column-major order; sixteen values are read ahead of their FMAs (a
+  // padding zero past the end leaves the sum unchanged: fma(0, 0, s) = s)
+  for (int pi = tid; pi < npairs; pi += NT) {
+    const int id = sm.pair_id[pi];
+    const int pr = id >> 4, pc = id & 15;
+    const int r0 = sm.rs_r[pr], m = sm.rs_r[pr + 1] - r0;
+    const int c0 = sm.rs_c[pc], n = sm.rs_c[pc + 1] - c0;
+    const int len = m * n;
+    double s = 0.0;
+    int r = 0, a = r0 * kPitch + c0;
+    for (int e0 = 0; e0 < len; e0 += 16) {
+      double x[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        x[u] = e0 + u < len ? Ct[a] : 0.0;
+        if (++r == m) {
+          r = 0;
+          a += 1 - (m - 1) * kPitch;
+        } else {
+          a += kPitch;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 16; ++u) s = __builtin_fma(x[u], x[u], s);
+    }
+    if (s >= eps2) atomicOr(&sm.keptw[pr], 1u << pc);
   }
   __syncthreads();
   if (tid < kBandAtoms) kept_out[(int64_t)t.tile * kBandAtoms + tid] = (uint16_t)sm.keptw[tid];

@@ -125,6 +125,66 @@ def test_identity_is_exact(ctx):
     check(ctx, atoms, a, eye, exact=True)
 
 
+def _lane_butterfly_nsq(x):
+    """Sum of squares of a block of at most 64 values in the order of a
+    lane-strided partial per lane followed by an xor-butterfly wave sum (the
+    order an epilogue with one wave per block would take)."""
+    v = np.zeros(64)
+    v[:len(x)] = x * x  # fma(x, x, 0) rounds x * x once
+    idx = np.arange(64)
+    for o in (32, 16, 8, 4, 2, 1):
+        v = v + v[idx ^ o]
+    return float(v[0])
+
+
+def _eps_between(lo, hi):
+    """An eps with lo < fl(eps * eps) <= hi, or None."""
+    e = float(np.sqrt(hi))
+    for _ in range(64):
+        e = float(np.nextafter(e, np.inf))
+    for _ in range(128):
+        if lo < e * e <= hi:
+            return e
+        e = float(np.nextafter(e, 0.0))
+    return None
+
+
+@pytest.mark.parametrize("direction", ["kept_by_sequential", "dropped_by_sequential"])
+def test_post_filter_norm_order_near_threshold(ctx, direction):
+    """C = I * B is exact, so only the summation order of nu^2(C_ij) decides a
+    block whose nu^2 lies within an ulp of eps^2. eps is chosen so that
+    fl(eps * eps) lies between the contract's sequential column-major FMA chain
+    and the lane-strided butterfly order: the GPU pattern must be the
+    oracle's (sequential) one, which the other order would not give."""
+    atoms, b = S.small_system(40, 7, sizes=(5,), cutoff=7.0, decay=0.3)
+    n = b.n_atoms
+    eye = S.BlockCSR(b.blk.copy(), np.arange(n + 1, dtype=np.int64),
+                     np.arange(n, dtype=np.int32), (25 * np.arange(n)).astype(np.int64),
+                     np.concatenate([np.eye(5).reshape(-1, order="F")] * n))
+    pick = None
+    for t in range(b.n_blocks):
+        x = b.data[b.blk_off[t]:b.blk_off[t] + 25]
+        seq, alt = oracle.nsq1(x), _lane_butterfly_nsq(x)
+        if direction == "kept_by_sequential" and alt < seq:
+            eps = _eps_between(alt, seq)
+        elif direction == "dropped_by_sequential" and seq < alt:
+            eps = _eps_between(seq, alt)
+        else:
+            continue
+        if eps is not None:
+            pick = (t, seq, alt, eps)
+            break
+    assert pick is not None, "no block with distinguishable summation orders"
+    t, seq, alt, eps = pick
+    print(f"case: block {t} seq={seq!r} alt={alt!r} eps^2={eps * eps!r}", flush=True)
+    st, rst = check(ctx, atoms, eye, b, eps=eps, exact=True)
+    ref, _ = oracle.multiply(eye, b, eps=eps)
+    i = int(np.searchsorted(b.row_p, t, side="right") - 1)
+    j = int(b.col[t])
+    cols = ref.col[ref.row_p[i]:ref.row_p[i + 1]]
+    assert (j in cols) == (direction == "kept_by_sequential")
+
+
 def test_integer_golden_is_exact(ctx):
     fx = np.load("tests/golden/integer_product.npz")
     blk = fx["blk"]
@@ -170,27 +230,48 @@ def test_upload_download_round_trip(ctx):
     dd.close()
 
 
-def test_autotuner_is_deterministic_and_variants_agree(monkeypatch):
-    """The tuning call (which times every FP64 variant) and later calls of the
-    same workload class return identical bits; each forced variant meets the
+def test_default_kernel_is_deterministic_and_variants_agree(monkeypatch):
+    """The FP64 variant is chosen by rule (half kernel on this sparse box) and
+    the autotuner only times tile orders, which permute the tile-to-workgroup
+    mapping: two contexts (two processes' worth of state), the tuning call and
+    the tuned calls all return identical bits. Each forced variant meets the
     oracle bar on its own."""
     atoms = S.water_system(240, seed=101)
     a = S.decay_matrix(atoms, 8.0, 0.4, 102)
     b = S.decay_matrix(atoms, 8.0, 0.4, 103)
     ref, _ = oracle.multiply(a, b, eps=1e-7)
-    fresh = D.Context(0)  # empty tuning cache
+    got = []
+    for _ in range(2):
+        fresh = D.Context(0)  # empty tuning cache
+        try:
+            g, _ = gpu_multiply(fresh, atoms, a, b, eps=1e-7)
+            v1, n1 = D.tuning_info(fresh)
+            g2, _ = gpu_multiply(fresh, atoms, a, b, eps=1e-7)
+            v2, n2 = D.tuning_info(fresh)
+            print(f"variant {v1}, tuned classes {n1}", flush=True)
+            assert (v1, n1) == (v2, n2) == (2, 1)
+            assert np.array_equal(g.data, g2.data)
+            got.append(g)
+        finally:
+            fresh.close()
+    assert np.array_equal(got[0].row_p, got[1].row_p)
+    assert np.array_equal(got[0].col, got[1].col)
+    assert np.array_equal(got[0].data, got[1].data)
+    for rb in ("4", "16"):
+        monkeypatch.setenv("LAMB_HIP_TILE_ROWBLOCK", rb)
+        fresh = D.Context(0)
+        try:
+            g, _ = gpu_multiply(fresh, atoms, a, b, eps=1e-7)
+            assert np.array_equal(g.data, got[0].data), rb
+        finally:
+            fresh.close()
+    monkeypatch.delenv("LAMB_HIP_TILE_ROWBLOCK")
+    fresh = D.Context(0)
     try:
-        got1, _ = gpu_multiply(fresh, atoms, a, b, eps=1e-7)
-        v1, n1 = D.tuning_info(fresh)
-        got2, _ = gpu_multiply(fresh, atoms, a, b, eps=1e-7)
-        v2, n2 = D.tuning_info(fresh)
-        print(f"autotuner chose variant {v1} ({n1} class)", flush=True)
-        assert (v1, n1) == (v2, n2) and n1 == 1
-        assert np.array_equal(got1.data, got2.data)
         for name in ("quadrant", "half", "ksplit"):
             monkeypatch.setenv("LAMB_HIP_F64_KERNEL", name)
-            got, _ = gpu_multiply(fresh, atoms, a, b, eps=1e-7)
-            same, err = compare(got, ref)
+            g, _ = gpu_multiply(fresh, atoms, a, b, eps=1e-7)
+            same, err = compare(g, ref)
             assert same and err <= TOL, (name, err)
     finally:
         fresh.close()
