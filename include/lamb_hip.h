@@ -173,6 +173,20 @@ int32_t lamb_hip_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A, const lamb_h
 /* tr(A): sum of the diagonal elements of the diagonal blocks (FP64). */
 int32_t lamb_hip_trace(const lamb_hip_mat* A, double* trace);
 
+/* tr(A) without allocating: no counterpart in the reference. For callers
+ * that cannot allocate (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc). The per-atom partial sums go to a fixed
+ * scratch array of the library, LAMB_HIP_NOALLOC_MAX_ATOMS doubles (8 MiB)
+ * on every GPU the library runs on, held until the process ends; A's
+ * distribution may hold at most that many atoms (more: LAMB_HIP_WRONG_ARGS).
+ * dev_trace is the device address of one double. The call enqueues its
+ * kernels on the context's stream and returns without waiting: dev_trace
+ * holds, once the stream gets there, the value lamb_hip_trace returns, bit
+ * for bit. Calls that overlap on one GPU (several contexts or host threads
+ * at once) share the scratch array and race. */
+#define LAMB_HIP_NOALLOC_MAX_ATOMS 1048576
+int32_t lamb_hip_trace_noalloc(const lamb_hip_mat* A, double* dev_trace);
+
 /* Kernel choice and autotuner state (no counterpart in the reference).
  * last_variant: the FP64 numeric kernel of the context's last multiply
  * (0 quadrant: per-wave 32 x 32 quadrants; 1 ksplit: whole-tile waves

@@ -84,6 +84,7 @@ SIGNATURES = {
                                       C.c_double, C.c_double, C.c_double, C.c_int32,
                                       C.POINTER(Stats)]),
     "lamb_hip_trace": (C.c_int32, [C.c_void_p, f64p]),
+    "lamb_hip_trace_noalloc": (C.c_int32, [C.c_void_p, C.c_void_p]),
     "lamb_hip_tuning_info": (C.c_int32, [C.c_void_p, i32p, i32p]),
     "lamb_hip_dist_set_groups": (C.c_int32, [C.c_void_p, C.c_int32, i32p]),
     "lamb_hip_group_trace": (C.c_int32, [C.c_void_p, f64p]),

@@ -91,7 +91,7 @@ int32_t lamb_hip_init(int32_t device, lamb_hip_ctx** out) {
       delete ctx;
       return LAMB_HIP_FATAL;
     }
-  if (install_guard(device, &ctx->guard) != hipSuccess) {
+  if (install_guard(device, &ctx->guard) != hipSuccess || noalloc_setup(device) != hipSuccess) {
     delete ctx;
     return LAMB_HIP_FATAL;
   }
@@ -218,6 +218,7 @@ int32_t lamb_hip_dist_create(lamb_hip_ctx* ctx, int64_t n_atoms, const int8_t* b
          "dist band_of");
   HIPCHK(ctx, d->d_pos_in_band.upload(ctx->stream, d->pos_in_band.data(), sizeof(int32_t) * d->n),
          "dist pos_in_band");
+  HIPCHK(ctx, d->d_perm.upload(ctx->stream, d->perm.data(), sizeof(int32_t) * d->n), "dist perm");
   std::vector<int64_t> elem0(d->n + 1, 0);  // alive until the stream sync below
   for (int32_t p = 0; p < d->n; ++p) elem0[p + 1] = elem0[p] + d->blk_int[p];
   d->n_elem = elem0[d->n];
@@ -239,6 +240,7 @@ int32_t lamb_hip_dist_destroy(lamb_hip_dist* d) {
   d->d_bands.release();
   d->d_band_of.release();
   d->d_pos_in_band.release();
+  d->d_perm.release();
   d->d_elem0.release();
   d->d_group.release();
   delete d;

@@ -60,6 +60,10 @@ int32_t refuse_missing_values(lamb_hip_ctx* ctx, const char* what);
 // Releases the device scratch of the symbolic phase held by a context.
 void release_sym_state(lamb_hip_ctx* ctx);
 
+// Device address of the no-allocation scratch array (noalloc.hip), taken once
+// per device with that device current.
+hipError_t noalloc_setup(int device);
+
 // Recursive coordinate bisection order of n atoms (coords: xyz per atom).
 void spatial_order(int32_t n, const float* coords, std::vector<int32_t>& order);
 
@@ -117,6 +121,7 @@ struct lamb_hip_dist {
   lamb::DevBuf d_blk;    // int32 by internal atom
   lamb::DevBuf d_bands;  // Band[]
   lamb::DevBuf d_band_of, d_pos_in_band;  // int32 by internal atom
+  lamb::DevBuf d_perm;   // int32 by global atom: its internal atom
   lamb::DevBuf d_elem0;  // int64 [n+1]: first matrix row of each internal atom
   int64_t n_elem = 0;    // matrix dimension (sum of block sizes)
   // groups of atoms (independent systems of a batch), by internal atom
