@@ -170,6 +170,23 @@ int32_t lamb_hip_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A, const lamb_h
                           lamb_hip_mat* C, double alpha, double beta, double eps_filter,
                           int32_t flags, lamb_hip_stats* stats);
 
+/* One sector product, C(m, n) += A(m, k) * B(k, n), for row subset m, inner
+ * subset k and column subset n (sector_multiply, src/lamb_multiplication.F90:
+ * 206-225, called from lamb_multiply's sector-pair loop :180-201, whose body
+ * is a stub in the reference). Triples are prefiltered as in
+ * lamb_hip_multiply (nsq(A_ik)*nsq(B_kj) >= eps_filter^2); C is an
+ * accumulator between sector products, so no post-filter runs here: apply
+ * lamb_hip_matrix_filter once every sector pair is in. C keeps every other
+ * sector unchanged. */
+int32_t lamb_hip_sector_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A,
+                                 const lamb_hip_mat* B, lamb_hip_mat* C, int32_t subset_m,
+                                 int32_t subset_k, int32_t subset_n, double eps_filter,
+                                 int32_t flags, lamb_hip_stats* stats);
+/* The post-filter of lamb_hip_multiply on its own: drops the blocks of C
+ * with nsq(C_ij) < eps_filter^2 (nsq as above); *n_dropped (may be NULL)
+ * gets their count. */
+int32_t lamb_hip_matrix_filter(lamb_hip_mat* C, double eps_filter, int64_t* n_dropped);
+
 /* tr(A): sum of the diagonal elements of the diagonal blocks (FP64). */
 int32_t lamb_hip_trace(const lamb_hip_mat* A, double* trace);
 
@@ -202,6 +219,37 @@ int32_t lamb_hip_trace_noalloc(const lamb_hip_mat* A, double* dev_trace);
  * every candidate writes the same bits. */
 int32_t lamb_hip_tuning_info(const lamb_hip_ctx* ctx, int32_t* last_variant,
                              int32_t* n_classes);
+
+/* --- sector ownership (plexus) -------------------------------------------
+ * The maps of lamb_plexus_calculate with force_n_procs
+ * (src/lamb_plexus_operations.F90:87-185) for process my_proc (0-based) of
+ * n_procs: region pools (fill_region_pools :198-320), the realm map that
+ * gives each sector (region pair) its owner (fill_realm_map :378-449), the
+ * process's catalog, orbit and sectors (fill_my_sectors :454-488) and its
+ * multiply schedule (lamb_plexus_create_sched :492-571). Regions and subsets
+ * are 1-based and one to one, as in the reference. n_regions = 0 takes the
+ * reference's count for n_procs (int((sqrt(1 + 8 P) - 1) / 2 * 2)), else it
+ * is forced (force_n_cubes). Host-side maps: no GPU call. Getters copy into
+ * caller arrays, any of which may be NULL:
+ *   realm_map [n_regions * n_regions], row-major (r - 1, s - 1);
+ *   pool_sizes [n_regions]; pools [n_regions * n_procs], row r - 1 holding
+ *   its sorted processes then -1; burdens [n_procs];
+ *   catalog [catalog_size], orbit [orbit_size] (sorted);
+ *   sectors [2 * n_sectors] as (row subset, col subset) pairs;
+ *   sched [6 * n_sched]: era, source process, this process, operand
+ *   (1 = A, 2 = B), row subset, col subset. */
+typedef struct lamb_hip_plexus lamb_hip_plexus;
+int32_t lamb_hip_plexus_create(int32_t n_procs, int32_t my_proc, int32_t n_regions,
+                               lamb_hip_plexus** plexus);
+int32_t lamb_hip_plexus_destroy(lamb_hip_plexus* plexus);
+int32_t lamb_hip_plexus_info(const lamb_hip_plexus* plexus, int32_t* n_regions,
+                             int32_t* catalog_size, int32_t* orbit_size, int32_t* n_sectors,
+                             int64_t* n_sched);
+int32_t lamb_hip_plexus_maps(const lamb_hip_plexus* plexus, int32_t* realm_map,
+                             int32_t* pool_sizes, int32_t* pools, int32_t* burdens);
+int32_t lamb_hip_plexus_lists(const lamb_hip_plexus* plexus, int32_t* catalog, int32_t* orbit,
+                              int32_t* sectors);
+int32_t lamb_hip_plexus_sched(const lamb_hip_plexus* plexus, int32_t* sched);
 
 /* --- batches and purification chains --------------------------------------
  * None of these has a counterpart in the reference, whose multiply computes

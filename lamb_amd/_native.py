@@ -85,6 +85,16 @@ SIGNATURES = {
                                       C.POINTER(Stats)]),
     "lamb_hip_trace": (C.c_int32, [C.c_void_p, f64p]),
     "lamb_hip_trace_noalloc": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "lamb_hip_sector_multiply": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                             C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                                             C.c_int32, C.POINTER(Stats)]),
+    "lamb_hip_matrix_filter": (C.c_int32, [C.c_void_p, C.c_double, i64p]),
+    "lamb_hip_plexus_create": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, vpp]),
+    "lamb_hip_plexus_destroy": (C.c_int32, [C.c_void_p]),
+    "lamb_hip_plexus_info": (C.c_int32, [C.c_void_p, i32p, i32p, i32p, i32p, i64p]),
+    "lamb_hip_plexus_maps": (C.c_int32, [C.c_void_p, i32p, i32p, i32p, i32p]),
+    "lamb_hip_plexus_lists": (C.c_int32, [C.c_void_p, i32p, i32p, i32p]),
+    "lamb_hip_plexus_sched": (C.c_int32, [C.c_void_p, i32p]),
     "lamb_hip_tuning_info": (C.c_int32, [C.c_void_p, i32p, i32p]),
     "lamb_hip_dist_set_groups": (C.c_int32, [C.c_void_p, C.c_int32, i32p]),
     "lamb_hip_group_trace": (C.c_int32, [C.c_void_p, f64p]),

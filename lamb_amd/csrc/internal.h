@@ -41,10 +41,12 @@ int32_t set_error(lamb_hip_ctx* ctx, int32_t level, int32_t type, const char* fm
 int32_t hip_error(lamb_hip_ctx* ctx, hipError_t e, const char* what);
 
 // lamb_hip_multiply restricted to the blocks of mask's pattern (mask may be
-// nullptr: the plain multiply).
+// nullptr: the plain multiply). post_eps >= 0 sets the post-filter threshold
+// apart from the prefilter's eps (0 keeps every block); < 0 uses eps.
 int32_t multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A, const lamb_hip_mat* B,
                         lamb_hip_mat* C, double alpha, double beta, double eps, int32_t flags,
-                        lamb_hip_stats* stats, const lamb_hip_mat* mask);
+                        lamb_hip_stats* stats, const lamb_hip_mat* mask,
+                        double post_eps = -1.0);
 
 // Per-atom partials (chain.hip): ||X - Y||_F^2 (Y may be nullptr) and the
 // diagonal sums, written to out[n_atoms] in internal atom order.

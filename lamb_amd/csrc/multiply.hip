@@ -172,7 +172,8 @@ extern "C" int32_t lamb_hip_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A,
 
 int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const lamb_hip_mat* B_,
                               lamb_hip_mat* C, double alpha, double beta, double eps,
-                              int32_t flags, lamb_hip_stats* stats, const lamb_hip_mat* mask) {
+                              int32_t flags, lamb_hip_stats* stats, const lamb_hip_mat* mask,
+                              double post_eps) {
   auto t_start = std::chrono::steady_clock::now();
   if (!ctx || !A_ || !B_ || !C) return LAMB_HIP_FAILURE;
   lamb_hip_mat* A = const_cast<lamb_hip_mat*>(A_);
@@ -210,6 +211,7 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
   if ((rc = ensure_nsq(A))) return rc;
   if ((rc = ensure_nsq(B))) return rc;
   const double eps2 = eps * eps;
+  const double post2 = post_eps < 0.0 ? eps2 : post_eps * post_eps;
   const bool use_old = beta != 0.0 && C->nblk > 0;
   SymState& S = sym_state(ctx);
   const DistView D = dist_view(d);
@@ -322,7 +324,7 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
       return launch_numeric_f64(s, n_tiles, (const TileDesc*)S.r.tiles.p,
                                 (const int32_t*)S.r.perm.p, (const Entry*)S.r.entries.p, D.bands,
                                 (const double*)ctx->d_apack.p, (const double*)ctx->d_bpack.p,
-                                a_zero, b_zero, alpha, beta, eps2,
+                                a_zero, b_zero, alpha, beta, post2,
                                 use_old ? MC.row_p : nullptr, use_old ? MC.col : nullptr,
                                 use_old ? MC.off : nullptr,
                                 use_old ? (const double*)C->d_data.p : nullptr,
@@ -330,7 +332,7 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
     return launch_numeric_bf16(s, slices, n_tiles, (const TileDesc*)S.r.tiles.p,
                                (const int32_t*)S.r.perm.p, (const Entry*)S.r.entries.p, D.bands,
                                ctx->d_aslice.p, ctx->d_bslice.p, a_zero, b_zero, alpha, beta,
-                               eps2, use_old ? MC.row_p : nullptr, use_old ? MC.col : nullptr,
+                               post2, use_old ? MC.row_p : nullptr, use_old ? MC.col : nullptr,
                                use_old ? MC.off : nullptr,
                                use_old ? (const double*)C->d_data.p : nullptr,
                                (double*)newdata.p, (uint16_t*)ctx->d_kept.p);
@@ -435,5 +437,137 @@ extern "C" int32_t lamb_hip_tuning_info(const lamb_hip_ctx* ctx, int32_t* last_v
   if (!ctx) return LAMB_HIP_FAILURE;
   if (last_variant) *last_variant = ctx->last_variant;
   if (n_classes) *n_classes = (int32_t)ctx->tune.size();
+  return LAMB_HIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// sector products and the post-filter on its own
+// ---------------------------------------------------------------------------
+namespace {
+// The blocks of X whose row atom lies in subset sr and column atom in subset
+// sc, as a temporary matrix with its own index that borrows X's value pool
+// (drop_view gives the pool back without freeing it).
+int32_t sector_view(lamb_hip_mat* X, int32_t sr, int32_t sc, lamb_hip_mat& V) {
+  lamb_hip_dist* d = X->dist;
+  lamb_hip_ctx* ctx = d->ctx;
+  HIPCHK(ctx, X->download_index(ctx->stream), "sector view: index download");
+  V.dist = d;
+  V.row_p.assign((size_t)d->n + 1, 0);
+  V.col.clear();
+  V.off.clear();
+  V.rows.clear();
+  int64_t ne = 0;
+  for (int32_t p = 0; p < d->n; ++p) {
+    if (d->subset_of[d->iperm[p]] == sr)
+      for (int64_t b = X->row_p[p]; b < X->row_p[p + 1]; ++b) {
+        const int32_t q = X->col[b];
+        if (d->subset_of[d->iperm[q]] != sc) continue;
+        V.col.push_back(q);
+        V.off.push_back(X->off[b]);
+        V.rows.push_back(p);
+        ne += (int64_t)d->blk_int[p] * d->blk_int[q];
+      }
+    V.row_p[p + 1] = (int64_t)V.col.size();
+  }
+  V.nblk = (int64_t)V.col.size();
+  V.ndata = X->ndata;
+  V.nelems = ne;
+  V.host_index_valid = true;
+  V.nsq_valid = false;
+  HIPCHK(ctx, V.upload_index(ctx->stream), "sector view: index upload");
+  V.d_data.p = X->d_data.p;
+  V.d_data.bytes = X->d_data.bytes;
+  return LAMB_HIP_OK;
+}
+
+void drop_view(lamb_hip_mat& V) {
+  V.d_data.p = nullptr;
+  V.d_data.bytes = 0;
+  V.release_device();
+}
+}  // namespace
+
+extern "C" int32_t lamb_hip_sector_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A,
+                                            const lamb_hip_mat* B, lamb_hip_mat* C,
+                                            int32_t sm, int32_t sk, int32_t sn, double eps,
+                                            int32_t flags, lamb_hip_stats* stats) {
+  if (!ctx || !A || !B || !C) return LAMB_HIP_FAILURE;
+  lamb_hip_dist* d = A->dist;
+  if (B->dist != d || C->dist != d)
+    return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_WRONG_ARGS,
+                     "sector_multiply: A, B, C must share one distribution");
+  if (C == A || C == B)
+    return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_WRONG_ARGS,
+                     "sector_multiply: C aliases A or B");
+  for (int32_t s : {sm, sk, sn})
+    if (s < 1 || s > d->n_subsets)
+      return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_WRONG_ARGS,
+                       "sector_multiply: subset %d outside 1..%d", s, d->n_subsets);
+  if (!(eps >= 0.0))
+    return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_WRONG_ARGS,
+                     "sector_multiply: eps_filter < 0");
+  if (values_missing(A) || values_missing(B) || values_missing(C))
+    return refuse_missing_values(ctx, "sector_multiply");
+  (void)hipSetDevice(ctx->device);
+  lamb_hip_mat VA, VB;
+  int32_t rc = sector_view(const_cast<lamb_hip_mat*>(A), sm, sk, VA);
+  if (!rc) rc = sector_view(const_cast<lamb_hip_mat*>(B), sk, sn, VB);
+  // C += A(m,k) B(k,n): beta = 1 keeps C's other blocks exactly (alpha * 0 +
+  // 1 * old), the post-filter threshold 0 keeps every block
+  if (!rc)
+    rc = multiply_masked(ctx, &VA, &VB, C, 1.0, C->nblk > 0 ? 1.0 : 0.0, eps, flags, stats,
+                         nullptr, 0.0);
+  drop_view(VA);
+  drop_view(VB);
+  return rc;
+}
+
+extern "C" int32_t lamb_hip_matrix_filter(lamb_hip_mat* C, double eps, int64_t* n_dropped) {
+  if (!C) return LAMB_HIP_FAILURE;
+  lamb_hip_ctx* ctx = C->dist->ctx;
+  if (!(eps >= 0.0))
+    return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_WRONG_ARGS, "matrix_filter: eps_filter < 0");
+  if (values_missing(C)) return refuse_missing_values(ctx, "matrix_filter");
+  (void)hipSetDevice(ctx->device);
+  int32_t rc;
+  if ((rc = ensure_nsq(C))) return rc;
+  HIPCHK(ctx, C->download_index(ctx->stream), "matrix_filter: index download");
+  std::vector<double> nsq((size_t)C->nblk);
+  if (C->nblk)
+    HIPCHK(ctx, hipMemcpyAsync(nsq.data(), C->d_nsq.p, sizeof(double) * nsq.size(),
+                               hipMemcpyDeviceToHost, ctx->stream), "matrix_filter: nsq");
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "matrix_filter sync");
+  const lamb_hip_dist* d = C->dist;
+  const double eps2 = eps * eps;
+  std::vector<int64_t> row_p((size_t)d->n + 1, 0);
+  std::vector<int32_t> col, rows;
+  std::vector<int64_t> off;
+  int64_t ne = 0, dropped = 0;
+  for (int32_t p = 0; p < d->n; ++p) {
+    for (int64_t b = C->row_p[p]; b < C->row_p[p + 1]; ++b) {
+      if (nsq[b] >= eps2) {
+        col.push_back(C->col[b]);
+        off.push_back(C->off[b]);
+        rows.push_back(p);
+        ne += (int64_t)d->blk_int[p] * d->blk_int[C->col[b]];
+      } else {
+        ++dropped;
+      }
+    }
+    row_p[p + 1] = (int64_t)col.size();
+  }
+  if (dropped) {
+    C->row_p = std::move(row_p);
+    C->col = std::move(col);
+    C->off = std::move(off);
+    C->rows = std::move(rows);
+    C->nblk = (int64_t)C->col.size();
+    C->nelems = ne;
+    C->host_index_valid = true;
+    C->invalidate_derived();
+    HIPCHK(ctx, C->upload_index(ctx->stream), "matrix_filter: index upload");
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream), "matrix_filter sync");
+  }
+  if (n_dropped) *n_dropped = dropped;
   return LAMB_HIP_OK;
 }

@@ -335,3 +335,77 @@ def multiply_test(ctx: Context, path, *, eps_filter: float = 0.0) -> dict:
                                         C.byref(st))
     ctx.check(rc, "multiply_test")
     return st.as_dict()
+
+
+def sector_multiply(a: DeviceMatrix, b: DeviceMatrix, c: DeviceMatrix, subset_m: int,
+                    subset_k: int, subset_n: int, *, eps_filter: float = 0.0,
+                    flags: int = FP64) -> dict:
+    """C(m, n) += A(m, k) * B(k, n) with the prefilter and no post-filter
+    (lamb_hip_sector_multiply; the reference's sector_multiply)."""
+    st = N.Stats()
+    rc = a.ctx.lib.lamb_hip_sector_multiply(a.ctx.h, a.h, b.h, c.h, subset_m, subset_k, subset_n,
+                                            eps_filter, flags, C.byref(st))
+    a.ctx.check(rc, f"sector_multiply({subset_m},{subset_k},{subset_n})")
+    return st.as_dict()
+
+
+def matrix_filter(c: DeviceMatrix, eps_filter: float) -> int:
+    """Drop C's blocks with nsq < eps_filter^2; returns how many were dropped."""
+    n = C.c_int64()
+    c.ctx.check(c.ctx.lib.lamb_hip_matrix_filter(c.h, eps_filter, C.byref(n)), "matrix_filter")
+    return n.value
+
+
+class Plexus:
+    """Sector ownership of process my_proc of n_procs (lamb_hip_plexus_*):
+    region pools, realm map, catalog, orbit, sectors and multiply schedule.
+    Regions / subsets are 1-based, processes 0-based."""
+
+    def __init__(self, n_procs: int, my_proc: int, n_regions: int = 0):
+        self.lib = N.load()
+        h = C.c_void_p()
+        rc = self.lib.lamb_hip_plexus_create(n_procs, my_proc, n_regions, C.byref(h))
+        if rc != 0:
+            raise LambError(rc, 100, f"plexus_create(n_procs={n_procs}, my_proc={my_proc}, "
+                                     f"n_regions={n_regions})")
+        self.h = h
+        self.n_procs = n_procs
+        self.my_proc = my_proc
+        nr, ncat, norb, nsec = (C.c_int32() for _ in range(4))
+        nsch = C.c_int64()
+        self.lib.lamb_hip_plexus_info(h, C.byref(nr), C.byref(ncat), C.byref(norb),
+                                      C.byref(nsec), C.byref(nsch))
+        self.n_regions = nr.value
+        n = self.n_regions
+        rmap = np.zeros(n * n, np.int32)
+        psz = np.zeros(n, np.int32)
+        pools = np.zeros(n * n_procs, np.int32)
+        burden = np.zeros(n_procs, np.int32)
+        self.lib.lamb_hip_plexus_maps(h, N.ptr(rmap, C.c_int32), N.ptr(psz, C.c_int32),
+                                      N.ptr(pools, C.c_int32), N.ptr(burden, C.c_int32))
+        self.realm_map = rmap.reshape(n, n)  # [r - 1, s - 1]
+        pools = pools.reshape(n, n_procs)
+        self.pools = {r + 1: pools[r, :psz[r]].tolist() for r in range(n)}
+        self.burden = burden.tolist()
+        cat = np.zeros(ncat.value, np.int32)
+        orb = np.zeros(norb.value, np.int32)
+        sec = np.zeros(2 * nsec.value, np.int32)
+        self.lib.lamb_hip_plexus_lists(h, N.ptr(cat, C.c_int32), N.ptr(orb, C.c_int32),
+                                       N.ptr(sec, C.c_int32))
+        self.catalog = cat.tolist()
+        self.orbit = orb.tolist()
+        self.sectors = [tuple(x) for x in sec.reshape(-1, 2).tolist()]
+        sch = np.zeros(6 * nsch.value, np.int32)
+        self.lib.lamb_hip_plexus_sched(h, N.ptr(sch, C.c_int32))
+        self.sched = [tuple(x) for x in sch.reshape(-1, 6).tolist()]
+
+    def close(self):
+        if self.h:
+            self.lib.lamb_hip_plexus_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
