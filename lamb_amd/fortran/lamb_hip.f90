@@ -45,6 +45,18 @@ module lamb_hip
     type(c_ptr) :: mat = c_null_ptr
   end type
 
+  ! lamb_hip_dbcsr_info, field for field
+  type, bind(C), public :: lamb_hip_dbcsr_info
+    character(kind=c_char) :: version(11)
+    character(kind=c_char) :: name(81)
+    character(kind=c_char) :: matrix_type
+    integer(c_int32_t) :: n_chunks
+    integer(c_int32_t) :: data_type
+    integer(c_int32_t) :: n_block_rows, n_block_cols
+    integer(c_int64_t) :: n_blocks
+    integer(c_int64_t) :: n_values
+  end type
+
   public :: lamb_hip_init, lamb_hip_finalize, lamb_hip_last_error
   public :: lamb_hip_dist_create, lamb_hip_dist_destroy
   public :: lamb_hip_matrix_create, lamb_hip_matrix_destroy, lamb_hip_matrix_copy
@@ -53,7 +65,7 @@ module lamb_hip
   public :: lamb_hip_goods_size, lamb_hip_goods_pack, lamb_hip_goods_unpack
   public :: lamb_hip_multiply, lamb_hip_trace, lamb_hip_dbcsr_load, lamb_hip_partition_atoms
   public :: lamb_multiply_hip, internal_distribute_hip, lamb_load_dbcsr_hip
-  public :: lamb_hip_multiply_test, lamb_multiply_test_hip
+  public :: lamb_hip_multiply_test, lamb_multiply_test_hip, lamb_hip_dbcsr_read_info
 
   interface
     function lamb_hip_init(device, ctx) bind(C, name="lamb_hip_init") result(st)
@@ -256,6 +268,17 @@ module lamb_hip
       integer(c_int32_t) :: st
     end function
 
+    function lamb_hip_dbcsr_read_info(path, info, row_blk_size, col_blk_size, msg, msg_len) &
+        bind(C, name="lamb_hip_dbcsr_read_info") result(st)
+      import :: c_int32_t, c_char, lamb_hip_dbcsr_info
+      character(kind=c_char), intent(in) :: path(*)
+      type(lamb_hip_dbcsr_info), intent(out) :: info
+      integer(c_int32_t), intent(out) :: row_blk_size(*), col_blk_size(*)
+      character(kind=c_char), intent(out) :: msg(*)
+      integer(c_int32_t), value :: msg_len
+      integer(c_int32_t) :: st
+    end function
+
     function lamb_hip_dbcsr_load(mat, path) bind(C, name="lamb_hip_dbcsr_load") result(st)
       import :: c_int32_t, c_ptr, c_char
       type(c_ptr), value :: mat
@@ -285,18 +308,22 @@ contains
     end do
   end subroutine
 
-  ! C = A*B with the norm filter eps_filter (alpha = 1, beta = 0: C is
-  ! overwritten), the argument order of lamb_multiply.
+  ! C = A*B (alpha = 1, beta = 0: C is overwritten) over device matrices, in
+  ! lamb_multiply's argument order; eps_filter (optional, default 0: no
+  ! filtering, as the reference's call has no threshold) sets the norm filter.
   subroutine lamb_multiply_hip(matrix_a, matrix_b, matrix_c, error, eps_filter, stats)
     type(lamb_hip_matrix), intent(in) :: matrix_a, matrix_b
     type(lamb_hip_matrix), intent(inout) :: matrix_c
     type(lamb_hip_error), intent(inout) :: error
-    real(c_double), intent(in) :: eps_filter
+    real(c_double), intent(in), optional :: eps_filter
     type(lamb_hip_stats), intent(out), optional :: stats
     type(lamb_hip_stats) :: st_local
+    real(c_double) :: eps
     integer(c_int32_t) :: st
+    eps = 0.0_c_double
+    if (present(eps_filter)) eps = eps_filter
     st = lamb_hip_multiply(matrix_c%ctx, matrix_a%mat, matrix_b%mat, matrix_c%mat, &
-                           1.0_c_double, 0.0_c_double, eps_filter, lamb_hip_fp64, st_local)
+                           1.0_c_double, 0.0_c_double, eps, lamb_hip_fp64, st_local)
     call fetch_error(matrix_c%ctx, st, error)
     if (present(stats)) stats = st_local
   end subroutine
