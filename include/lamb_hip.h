@@ -219,6 +219,9 @@ int32_t lamb_hip_trace_noalloc(const lamb_hip_mat* A, double* dev_trace);
  * every candidate writes the same bits. */
 int32_t lamb_hip_tuning_info(const lamb_hip_ctx* ctx, int32_t* last_variant,
                              int32_t* n_classes);
+/* Row bands per block of the tile order the context's last multiply used (the
+ * autotuner's choice for its workload class, or LAMB_HIP_TILE_ROWBLOCK). */
+int32_t lamb_hip_tuning_order(const lamb_hip_ctx* ctx, int32_t* last_rowblock);
 
 /* --- sector ownership (plexus) -------------------------------------------
  * The maps of lamb_plexus_calculate with force_n_procs

@@ -571,3 +571,9 @@ extern "C" int32_t lamb_hip_matrix_filter(lamb_hip_mat* C, double eps, int64_t* 
   if (n_dropped) *n_dropped = dropped;
   return LAMB_HIP_OK;
 }
+
+extern "C" int32_t lamb_hip_tuning_order(const lamb_hip_ctx* ctx, int32_t* last_rowblock) {
+  if (!ctx) return LAMB_HIP_FAILURE;
+  if (last_rowblock) *last_rowblock = ctx->last_rowblock;
+  return LAMB_HIP_OK;
+}

@@ -246,6 +246,13 @@ def partition_atoms(ctx: Context, coords, box, dims):
     return subset, order
 
 
+def tuning_order(ctx: Context) -> int:
+    """Row bands per block of the tile order of the context's last multiply."""
+    rb = C.c_int32()
+    ctx.check(ctx.lib.lamb_hip_tuning_order(ctx.h, C.byref(rb)), "tuning_order")
+    return rb.value
+
+
 def tuning_info(ctx: Context):
     """(FP64 kernel variant of the last multiply, workload classes tuned)."""
     v, n = C.c_int32(), C.c_int32()
