@@ -56,6 +56,13 @@ def parse():
                    help="config4: largest system (atoms). Iterates fill in to dense after one "
                         "McWeeny step; with systems up to 3,072 atoms a dense iterate of the "
                         "64-system batch is 78 GB and a step does not fit one GPU")
+    p.add_argument("--mem-groups", action="store_true",
+                   help="config4: run the batch in groups of systems sized to device memory, "
+                        "each group's chain captured as one graph (for the full 96..3,072 range)")
+    p.add_argument("--replays", type=int, default=1,
+                   help="config4 --mem-groups: graph replays per group (compared bit for bit)")
+    p.add_argument("--mem-fraction", type=float, default=0.8,
+                   help="config4 --mem-groups: share of free device memory a group may use")
     p.add_argument("--chain-steps", type=int, default=0,
                    help="McWeeny steps (default 20 for config4, 10 for config5)")
     p.add_argument("--atoms", type=int, default=3072, help="config5: atoms of the water box")
@@ -197,6 +204,108 @@ def batch_distribution(atoms, Distribution):
     raise ValueError("no spatial partition keeps every subset within the int16 local index")
 
 
+# Device bytes per dense-iterate byte of a captured McWeeny chain: the
+# iterate, its square and the update with a spare buffer each (6), the
+# captured input and the two packed operand panels (about 2.5), the tile
+# entries of a dense product (up to about 2 for 3,000-atom systems), and
+# headroom for the symbolic scratch.
+CHAIN_BYTES_PER_DENSE_BYTE = 12.0
+
+
+def memory_groups(systems, budget):
+    """Systems split into groups whose estimated chain memory fits budget
+    bytes: largest first, each into the first group with room (first-fit
+    decreasing). Returns lists of indices into systems."""
+    need = []
+    for atoms, _ in systems:
+        dim = float(np.sum(atoms.blk))
+        need.append(CHAIN_BYTES_PER_DENSE_BYTE * 8.0 * dim * dim)
+    groups, room = [], []
+    for i in sorted(range(len(systems)), key=lambda i: -need[i]):
+        for g in range(len(groups)):
+            if need[i] <= room[g]:
+                groups[g].append(i)
+                room[g] -= need[i]
+                break
+        else:
+            if need[i] > budget:
+                raise ValueError(f"system {i} needs {need[i] / 1e9:.1f} GB, above the "
+                                 f"{budget / 1e9:.1f} GB budget")
+            groups.append([i])
+            room.append(budget - need[i])
+    return [sorted(g) for g in groups], need
+
+
+def run_config4_groups(args, bt, steps, eps):
+    """The batch in memory-bounded groups: per group one distribution, the
+    chain captured as a HIP graph (its record pass is the eager chain) and
+    replayed --replays times; flops and the fastest replay of each group summed
+    over groups."""
+    import torch
+    from lamb_amd import device as D
+    from lamb_amd import systems as S
+    from lamb_amd.matrix import Distribution, LambMatrix
+    free, total = torch.cuda.mem_get_info(0)
+    budget = args.mem_fraction * free
+    groups, need = memory_groups(bt.systems, budget)
+    print(f"{len(groups)} groups within {budget / 1e9:.0f} GB of {free / 1e9:.0f} GB free: "
+          f"{[len(g) for g in groups]} systems", file=sys.stderr, flush=True)
+    n = bt.n_systems
+    n_classes = 0
+    final_idem = np.full(n, np.nan)
+    final_trace = np.full(n, np.nan)
+    flops = ms_replay = ms_capture = 0.0
+    replays_equal = True
+    per_group = []
+    for gi, idx in enumerate(groups):
+        gb = S.batch([bt.systems[i] for i in idx])
+        dist = batch_distribution(gb.atoms, Distribution)
+        ctx = D.Context(0)  # a context per group: its scratch goes with it
+        dd = D.DeviceDistribution(ctx, dist)
+        dd.set_groups(gb.group, gb.n_systems)
+        dh = D.DeviceMatrix.from_lamb(dd, LambMatrix.from_global(dist, gb.h))
+        p0 = D.purification_start(dh)
+        t0 = time.perf_counter()
+        g = D.ChainGraph(p0, steps, eps_filter=eps)
+        t_cap = (time.perf_counter() - t0) * 1e3
+        out = D.DeviceMatrix(dd)
+        reps = [g.replay(p0, out) for _ in range(max(args.replays, 1))]
+        tr2, id2, _ = reps[-1]
+        replays_equal &= all(np.array_equal(r[0], tr2) and np.array_equal(r[1], id2)
+                             for r in reps)
+        f = g.info()["flops"]
+        ms = min(r[2] for r in reps)
+        flops += f
+        ms_replay += ms
+        ms_capture += t_cap
+        final_idem[idx] = id2[-1]
+        final_trace[idx] = tr2[-1]
+        per_group.append({"systems": len(idx), "atoms": int(gb.atoms.n_atoms),
+                          "flops": f, "ms_replay": ms, "ms_capture": t_cap})
+        print(f"group {gi + 1}/{len(groups)}: {len(idx)} systems, {gb.atoms.n_atoms} atoms, "
+              f"est. {sum(need[i] for i in idx) / 1e9:.0f} GB; capture {t_cap:.0f} ms, replay "
+              f"{ms:.0f} ms, {f / (ms * 1e-3) / 1e12:.1f} TFLOP/s", file=sys.stderr, flush=True)
+        for h in (g, out, p0, dh, dd):
+            h.close()
+        n_classes += D.tuning_info(ctx)[1]
+        ctx.close()
+    sizes = [a.n_atoms for a, _ in bt.systems]
+    return line("batched McWeeny chains: GFLOP/s of the chains replayed as HIP graphs, in "
+                "memory-bounded groups", flops / (ms_replay * 1e-3) / 1e9, "GFLOP/s", ms_replay,
+                args,
+                {"workload": f"{n} water-like systems, {min(sizes)}..{max(sizes)} atoms"
+                             f" (log-uniform, seed 4), O 13 / H 5 / 23 for 10%, r<8 A, "
+                             f"{steps} McWeeny steps from H scaled to [0,1], eps 1e-7, in "
+                             f"{len(groups)} groups sized to device memory",
+                 "atoms": int(bt.atoms.n_atoms), "chain_steps": steps, "groups": per_group,
+                 "flops_per_chain": flops, "ms_graph_replay": ms_replay,
+                 "ms_capture_incl_record": ms_capture, "replays_equal_bitwise": replays_equal,
+                 "final_idempotency_error": final_idem.tolist(),
+                 "final_trace": final_trace.tolist(),
+                 "max_final_idempotency_error": float(np.nanmax(final_idem)),
+                 "autotuned_classes": n_classes})
+
+
 def run_config4(args):
     """A batch of independent systems, McWeeny chains eager and as a HIP graph."""
     from lamb_amd import device as D
@@ -209,6 +318,8 @@ def run_config4(args):
     sizes = [a.n_atoms for a, _ in bt.systems]
     print(f"batch of {bt.n_systems} systems, {bt.atoms.n_atoms} atoms ({min(sizes)}..{max(sizes)}),"
           f" built in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    if args.mem_groups:
+        return run_config4_groups(args, bt, steps, eps)
     dist = batch_distribution(bt.atoms, Distribution)
     ctx = D.Context(0)
     dd = D.DeviceDistribution(ctx, dist)

@@ -4,14 +4,16 @@
 // (SDDMM: only blocks stored in the operand are computed). Each is one
 // masked multiply by an explicit transpose, so it runs on the same MFMA tile
 // kernels (FP64 or BF16 split) as the forward product. Transposes build their
-// index on the host (stable: rows of the transpose keep ascending columns)
-// and move the block data on the device. Values can be read and written in a
+// index and move the block data on the device (a stable sort by column, so
+// rows of the transpose keep ascending columns). Values can be read and written in a
 // canonical order (blocks in CSR order of the internal atom order, each
 // column-major, back to back) through device pointers.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <vector>
+
+#include <hipcub/hipcub.hpp>
 
 #include "common.h"
 #include "device_util.h"
@@ -73,6 +75,44 @@ __global__ __launch_bounds__(256) void project_blocks_kernel(
   }
 }
 
+__global__ void iota_i32_kernel(int64_t n, int32_t* __restrict__ out) {
+  LAMB_GUARD;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) out[t] = (int32_t)t;
+}
+
+// position p of XT in CSR order: X's block ids[p] = (i, j) becomes XT's
+// (j, i); its source offset and element count
+__global__ __launch_bounds__(256) void transpose_gather_kernel(
+    int64_t nblk, const int32_t* __restrict__ ids, const int32_t* __restrict__ t_rows,
+    const int32_t* __restrict__ x_rows, const int64_t* __restrict__ x_off,
+    const int32_t* __restrict__ blk, int32_t* __restrict__ t_col, int64_t* __restrict__ src_off,
+    int64_t* __restrict__ sz) {
+  LAMB_GUARD;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nblk) return;
+  const int32_t b = ids[p];
+  const int32_t i = x_rows[b];
+  t_col[p] = i;
+  src_off[p] = x_off[b];
+  sz[p] = (int64_t)blk[i] * blk[t_rows[p]];
+}
+
+// row_p[j] = first position whose row is >= j, over XT's sorted rows
+__global__ void transpose_row_p_kernel(int32_t n, int64_t nblk, const int32_t* __restrict__ t_rows,
+                                       int64_t* __restrict__ row_p) {
+  LAMB_GUARD;
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j > n) return;
+  int64_t lo = 0, hi = nblk;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (t_rows[mid] < j) lo = mid + 1;
+    else hi = mid;
+  }
+  row_p[j] = lo;
+}
+
 __global__ void block_sizes_kernel(int64_t nblk, const int32_t* __restrict__ rows,
                                    const int32_t* __restrict__ col,
                                    const int32_t* __restrict__ blk, int64_t* __restrict__ sz) {
@@ -112,57 +152,79 @@ namespace {
 
 unsigned grid1(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + 255) / 256); }
 
-// XT = X^T (same distribution)
+// XT = X^T (same distribution), index and values built on the device. X's
+// blocks in CSR order have ascending rows, so a stable radix sort of them by
+// column lists XT's blocks in CSR order with each row's columns ascending.
 int32_t transpose_into(lamb_hip_mat* X, lamb_hip_mat* XT) {
   lamb_hip_dist* d = X->dist;
   lamb_hip_ctx* ctx = d->ctx;
   if (values_missing(X)) return refuse_missing_values(ctx, "transpose");
   hipStream_t s = ctx->stream;
-  HIPCHK(ctx, X->download_index(s), "transpose: index download");
   const int32_t n = d->n;
   const int64_t nb = X->nblk;
-  std::vector<int64_t> row_p(n + 1, 0);
-  for (int64_t b = 0; b < nb; ++b) ++row_p[X->col[b] + 1];
-  for (int32_t j = 0; j < n; ++j) row_p[j + 1] += row_p[j];
-  std::vector<int64_t> cur(row_p.begin(), row_p.end() - 1);
-  std::vector<int32_t> col(nb), rows(nb);
-  std::vector<int64_t> src_off(nb), off(nb);
-  for (int32_t i = 0; i < n; ++i)
-    for (int64_t b = X->row_p[i]; b < X->row_p[i + 1]; ++b) {
-      const int32_t j = X->col[b];
-      const int64_t pos = cur[j]++;
-      col[pos] = i;
-      rows[pos] = j;
-      src_off[pos] = X->off[b];
-    }
+  if (nb > INT32_MAX)
+    return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_OVERFLOW_ERROR,
+                     "transpose: %lld blocks exceed the sort's 32-bit count", (long long)nb);
+  const size_t cap = (size_t)std::max<int64_t>(nb, 1);
+  HIPCHK(ctx, XT->d_row_p.ensure(sizeof(int64_t) * (size_t)(n + 1)), "transpose: row_p");
+  HIPCHK(ctx, XT->d_rows.ensure(sizeof(int32_t) * cap), "transpose: rows");
+  HIPCHK(ctx, XT->d_col.ensure(sizeof(int32_t) * cap), "transpose: col");
+  HIPCHK(ctx, XT->d_off.ensure(sizeof(int64_t) * (cap + 1)), "transpose: off");
+  DevBuf ids, sorted_ids, src_off, sz, scr, temp;
+  HIPCHK(ctx, ids.ensure(sizeof(int32_t) * cap), "transpose: ids");
+  HIPCHK(ctx, sorted_ids.ensure(sizeof(int32_t) * cap), "transpose: ids");
+  HIPCHK(ctx, src_off.ensure(sizeof(int64_t) * cap), "transpose: offsets");
+  HIPCHK(ctx, sz.ensure(sizeof(int64_t) * cap), "transpose: sizes");
+  HIPCHK(ctx, scr.ensure(sizeof(int64_t) * (size_t)scan_scratch_elems(nb)), "transpose: scan");
   int64_t total = 0;
-  for (int64_t b = 0; b < nb; ++b) {
-    off[b] = total;
-    total += (int64_t)d->blk_int[rows[b]] * d->blk_int[col[b]];
+  if (nb) {
+    int end_bit = 1;
+    while (end_bit < 31 && ((int64_t)1 << end_bit) < n) ++end_bit;
+    hipLaunchKernelGGL(iota_i32_kernel, dim3(grid1(nb)), dim3(256), 0, s, nb, (int32_t*)ids.p);
+    HIPCHK(ctx, hipGetLastError(), "transpose: ids kernel");
+    size_t temp_bytes = 0;
+    HIPCHK(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, (const int32_t*)X->d_col.p,
+                                                   (int32_t*)XT->d_rows.p, (const int32_t*)ids.p,
+                                                   (int32_t*)sorted_ids.p, (int)nb, 0, end_bit, s),
+           "transpose: sort size");
+    HIPCHK(ctx, temp.ensure(std::max<size_t>(temp_bytes, 8)), "transpose: sort scratch");
+    HIPCHK(ctx, hipcub::DeviceRadixSort::SortPairs(temp.p, temp_bytes, (const int32_t*)X->d_col.p,
+                                                   (int32_t*)XT->d_rows.p, (const int32_t*)ids.p,
+                                                   (int32_t*)sorted_ids.p, (int)nb, 0, end_bit, s),
+           "transpose: sort");
+    hipLaunchKernelGGL(transpose_gather_kernel, dim3(grid1(nb)), dim3(256), 0, s, nb,
+                       (const int32_t*)sorted_ids.p, (const int32_t*)XT->d_rows.p,
+                       (const int32_t*)X->d_rows.p, (const int64_t*)X->d_off.p,
+                       (const int32_t*)d->d_blk.p, (int32_t*)XT->d_col.p, (int64_t*)src_off.p,
+                       (int64_t*)sz.p);
+    HIPCHK(ctx, hipGetLastError(), "transpose: gather kernel");
+    HIPCHK(ctx, exclusive_scan(s, (const int64_t*)sz.p, (int64_t*)XT->d_off.p, nb,
+                               (int64_t*)scr.p), "transpose: offsets scan");
+    HIPCHK(ctx, hipMemcpyAsync(&total, (const int64_t*)XT->d_off.p + nb, sizeof(int64_t),
+                               hipMemcpyDeviceToHost, s), "transpose: size read");
   }
-  XT->row_p = std::move(row_p);
-  XT->col = std::move(col);
-  XT->rows = std::move(rows);
-  XT->off = std::move(off);
+  hipLaunchKernelGGL(transpose_row_p_kernel, dim3(grid1(n + 1)), dim3(256), 0, s, n, nb,
+                     (const int32_t*)XT->d_rows.p, (int64_t*)XT->d_row_p.p);
+  HIPCHK(ctx, hipGetLastError(), "transpose: row_p kernel");
+  HIPCHK(ctx, hipStreamSynchronize(s), "transpose: index sync");
   XT->nblk = nb;
   XT->ndata = total;
   XT->nelems = total;
-  XT->host_index_valid = true;
+  XT->host_index_valid = false;
   XT->invalidate_derived();
-  HIPCHK(ctx, XT->upload_index(s), "transpose: index upload");
+  XT->host_data_valid = false;
+  XT->staged_sectors.clear();
+  XT->staged_lists.clear();
   HIPCHK(ctx, XT->d_data.ensure(sizeof(double) * (size_t)std::max<int64_t>(total, 1)),
          "transpose: data");
-  DevBuf so;
-  HIPCHK(ctx, so.upload(s, src_off.data(), sizeof(int64_t) * (size_t)nb), "transpose: offsets");
   if (nb)
     hipLaunchKernelGGL(transpose_blocks_kernel, dim3(grid1(nb)), dim3(256), 0, s, nb,
                        (const int32_t*)XT->d_rows.p, (const int32_t*)XT->d_col.p,
-                       (const int64_t*)XT->d_off.p, (const int64_t*)so.p,
+                       (const int64_t*)XT->d_off.p, (const int64_t*)src_off.p,
                        (const int32_t*)d->d_blk.p, (const double*)X->d_data.p,
                        (double*)XT->d_data.p);
   HIPCHK(ctx, hipGetLastError(), "transpose kernel");
   HIPCHK(ctx, hipStreamSynchronize(s), "transpose sync");
-  so.release();
   return LAMB_HIP_OK;
 }
 

@@ -17,6 +17,7 @@ import pytest
 import torch
 
 import oracle
+from __graft_entry__ import compare
 from lamb_amd import _native as N
 from lamb_amd import autograd as AG
 from lamb_amd import device as D
@@ -56,6 +57,66 @@ def test_adjoint_matches_oracle(ctx, flags, tol):
     print(f"  dA error {ea:.3e}, dB error {eb:.3e}", flush=True)
     assert ea <= tol and eb <= tol
     for x in (dA, dB, dg_, db_, da_, dd):
+        x.close()
+
+
+def transposed(m: S.BlockCSR) -> S.BlockCSR:
+    """X^T in numpy: block (i, j) becomes (j, i), transposed, rows' columns
+    ascending."""
+    rows = m.rows()
+    order = np.lexsort((rows, m.col))  # by new row (old col), then new col
+    sizes = m.blk[rows].astype(np.int64) * m.blk[m.col]
+    t_rows, t_col = m.col[order], rows[order]
+    t_off = np.concatenate([[0], np.cumsum(sizes[order])[:-1]]).astype(np.int64)
+    data = np.empty_like(m.data)
+    for p, b in enumerate(order):
+        blk = m.block(int(b))  # m_i x n_j
+        data[t_off[p]:t_off[p] + blk.size] = blk.T.reshape(-1, order="F")
+    row_p = np.zeros(m.n_atoms + 1, dtype=np.int64)
+    np.add.at(row_p, t_rows.astype(np.int64) + 1, 1)
+    return S.BlockCSR(m.blk, np.cumsum(row_p), t_col.astype(np.int32), t_off, data)
+
+
+@pytest.mark.parametrize("n_procs", [1, 4])
+def test_transpose_on_device(ctx, n_procs):
+    """lamb_hip_matrix_transpose (index by a stable sort on the device):
+    exactly X^T, pattern and values, against numpy; twice gives X back bit
+    for bit; the transpose is a valid operand (X^T B against the oracle); an
+    empty matrix transposes to an empty one."""
+    atoms = S.water_system(150, seed=65, heavy_fraction=0.1)
+    a = S.decay_matrix(atoms, 7.0, 0.4, 66)
+    b = S.decay_matrix(atoms, 6.0, 0.4, 67)
+    want = transposed(a)
+    print(f"case: transpose atoms={atoms.n_atoms} blocks={a.n_blocks} n_procs={n_procs}",
+          flush=True)
+    dist = Distribution(atoms.blk, atoms.pos, atoms.box, n_procs=n_procs)
+    dd = D.DeviceDistribution(ctx, dist)
+    da = D.DeviceMatrix.from_lamb(dd, LambMatrix.from_global(dist, a))
+    db = D.DeviceMatrix.from_lamb(dd, LambMatrix.from_global(dist, b))
+    dt, dtt = D.DeviceMatrix(dd), D.DeviceMatrix(dd)
+    ctx.check(ctx.lib.lamb_hip_matrix_transpose(da.h, dt.h), "transpose")
+    got = dt.download().to_global()
+    assert np.array_equal(got.row_p, want.row_p) and np.array_equal(got.col, want.col)
+    assert np.array_equal(got.to_dense(), a.to_dense().T)
+    assert np.array_equal(got.data, want.data)
+    ctx.check(ctx.lib.lamb_hip_matrix_transpose(dt.h, dtt.h), "transpose")
+    back = dtt.download().to_global()
+    ref_a = da.download().to_global()
+    assert np.array_equal(back.row_p, ref_a.row_p) and np.array_equal(back.col, ref_a.col)
+    assert np.array_equal(back.data, ref_a.data)
+    dc = D.DeviceMatrix(dd)
+    D.multiply(dt, db, dc, eps_filter=1e-7)
+    ref, _ = oracle.multiply(want, b, eps=1e-7)
+    same, err = compare(dc.download().to_global(), ref)
+    print(f"  X^T B vs oracle: pattern {same}, error {err:.3e}", flush=True)
+    assert same and err <= 1e-12
+    empty = S.BlockCSR(a.blk, np.zeros(a.n_atoms + 1, dtype=np.int64), np.zeros(0, np.int32),
+                       np.zeros(0, np.int64), np.zeros(0))
+    de = D.DeviceMatrix.from_lamb(dd, LambMatrix.from_global(dist, empty))
+    ctx.check(ctx.lib.lamb_hip_matrix_transpose(de.h, dt.h), "transpose (empty)")
+    got = dt.download().to_global()
+    assert got.n_blocks == 0 and not got.row_p.any()
+    for x in (de, dc, dtt, dt, db, da, dd):
         x.close()
 
 

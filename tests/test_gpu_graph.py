@@ -5,8 +5,9 @@ path computes: the resulting matrix, traces and idempotency errors must be
 bit-identical to the eager chain's (same kernels in the same order), on two
 replays in a row, in FP64 and BF16-split form. A replay whose input values
 change the filter decisions must fail cleanly (precondition error) and
-leave the library usable; the values are scaled down so that every size of
-that replay is smaller than the recorded one.
+leave the library usable: once with the values scaled down, so that the
+replay's sizes are smaller than the recorded ones, and once scaled up under
+a binding filter, so that they are larger.
 """
 import numpy as np
 import pytest
@@ -65,4 +66,45 @@ def test_replay_with_other_filter_decisions_fails_cleanly(ctx):
     g.replay(p0, out)
     D.mcweeny(small, 2, eps_filter=1e-7)
     for x in (g, out, small, p0, dh, dd):
+        x.close()
+
+
+def test_replay_with_growing_sizes_fails_cleanly(ctx):
+    """Values scaled up with a binding filter: the replay's products keep more
+    triples and C blocks than the recorded chain, so its sizes exceed the
+    buffers captured in the graph. The size check at the first differing read
+    stops every later kernel before it writes through the larger size: the
+    replay fails with a precondition error, and the graph afterwards still
+    computes the eager chain bit for bit (nothing it holds was overwritten)."""
+    b, dd, dh = batch_h(ctx, 23)
+    steps, eps = 3, 1e-3
+    p0 = D.purification_start(dh)
+    big = D.DeviceMatrix(dd)
+    big.copy_from(p0)
+    big.group_scale_shift(10.0, 0.0)  # stronger blocks: the filters keep more
+    c = D.DeviceMatrix(dd)
+    s0 = D.multiply(p0, p0, c, eps_filter=eps)
+    s1 = D.multiply(big, big, c, eps_filter=eps)
+    print(f"case: growing replay, atoms={b.atoms.n_atoms}, eps {eps}: first product "
+          f"triples {s0['n_triples_kept']} -> {s1['n_triples_kept']}, C blocks "
+          f"{s0['n_c_blocks_kept']} -> {s1['n_c_blocks_kept']}", flush=True)
+    assert s1["n_triples_kept"] > s0["n_triples_kept"]
+    assert s1["n_c_blocks_kept"] > s0["n_c_blocks_kept"]
+    pe = D.DeviceMatrix(dd)
+    pe.copy_from(p0)
+    te, ie, _ = D.mcweeny(pe, steps, eps_filter=eps)
+    want = pe.download().to_global()
+    g = D.ChainGraph(p0, steps, eps_filter=eps)
+    out = D.DeviceMatrix(dd)
+    with pytest.raises(D.LambError) as err:
+        g.replay(big, out)
+    assert err.value.type == 200  # precondition
+    for rep in range(2):
+        tg, ig, _ = g.replay(p0, out)
+        got = out.download().to_global()
+        print(f"  replay {rep} of the captured input after the failed one", flush=True)
+        assert np.array_equal(got.row_p, want.row_p) and np.array_equal(got.col, want.col)
+        assert np.array_equal(got.data, want.data)
+        assert np.array_equal(tg, te) and np.array_equal(ig, ie)
+    for x in (g, out, pe, c, big, p0, dh, dd):
         x.close()
