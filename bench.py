@@ -61,8 +61,10 @@ def parse():
                         "each group's chain captured as one graph (for the full 96..3,072 range)")
     p.add_argument("--replays", type=int, default=1,
                    help="config4 --mem-groups: graph replays per group (compared bit for bit)")
-    p.add_argument("--mem-fraction", type=float, default=0.8,
-                   help="config4 --mem-groups: share of free device memory a group may use")
+    p.add_argument("--mem-fraction", type=float, default=0.4,
+                   help="config4 --mem-groups: share of free device memory a group may use "
+                        "(by the estimate CHAIN_BYTES_PER_DENSE_BYTE; smaller groups also keep "
+                        "each group's chain under a minute)")
     p.add_argument("--chain-steps", type=int, default=0,
                    help="McWeeny steps (default 20 for config4, 10 for config5)")
     p.add_argument("--atoms", type=int, default=3072, help="config5: atoms of the water box")
@@ -268,6 +270,8 @@ def run_config4_groups(args, bt, steps, eps):
         t0 = time.perf_counter()
         g = D.ChainGraph(p0, steps, eps_filter=eps)
         t_cap = (time.perf_counter() - t0) * 1e3
+        print(f"group {gi + 1}/{len(groups)}: {len(idx)} systems, {gb.atoms.n_atoms} atoms; "
+              f"recorded and captured in {t_cap:.0f} ms", file=sys.stderr, flush=True)
         out = D.DeviceMatrix(dd)
         reps = [g.replay(p0, out) for _ in range(max(args.replays, 1))]
         tr2, id2, _ = reps[-1]
@@ -282,9 +286,9 @@ def run_config4_groups(args, bt, steps, eps):
         final_trace[idx] = tr2[-1]
         per_group.append({"systems": len(idx), "atoms": int(gb.atoms.n_atoms),
                           "flops": f, "ms_replay": ms, "ms_capture": t_cap})
-        print(f"group {gi + 1}/{len(groups)}: {len(idx)} systems, {gb.atoms.n_atoms} atoms, "
-              f"est. {sum(need[i] for i in idx) / 1e9:.0f} GB; capture {t_cap:.0f} ms, replay "
-              f"{ms:.0f} ms, {f / (ms * 1e-3) / 1e12:.1f} TFLOP/s", file=sys.stderr, flush=True)
+        print(f"group {gi + 1}/{len(groups)}: est. {sum(need[i] for i in idx) / 1e9:.0f} GB; "
+              f"replay {ms:.0f} ms, {f / (ms * 1e-3) / 1e12:.1f} TFLOP/s", file=sys.stderr,
+              flush=True)
         for h in (g, out, p0, dh, dd):
             h.close()
         n_classes += D.tuning_info(ctx)[1]
