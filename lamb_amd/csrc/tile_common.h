@@ -17,7 +17,8 @@ struct TileSmem {
   int32_t wave_cnt[4];
   int64_t wave_sz[4];
   uint32_t keptw[kBandAtoms];
-  uint8_t r8_r[kBandAtoms], r8_c[kBandAtoms];  // 8-row / 8-col regions per atom
+  int32_t max_rows, max_cols;  // largest row / column atom of the tile
+  double zero;                 // read by norm-chain slots outside a block
 };
 
 struct TileInfo {
@@ -62,6 +63,11 @@ __device__ __forceinline__ TileInfo tile_setup(const TileDesc* __restrict__ tile
     sm.rs_c[tid] = bands[t.J].rs[min(tid, t.natJ)];
   }
   if (tid < kBandAtoms) sm.keptw[tid] = 0;
+  if (tid == 0) {
+    sm.max_rows = 0;
+    sm.max_cols = 0;
+    sm.zero = 0.0;
+  }
   __syncthreads();
   if (tid < kBandAtoms) {
     uint32_t g = 0;
@@ -72,13 +78,8 @@ __device__ __forceinline__ TileInfo tile_setup(const TileDesc* __restrict__ tile
     if (tid < t.natJ && sm.rs_c[tid + 1] > sm.rs_c[tid])
       for (int q = sm.rs_c[tid] / 4; q <= (sm.rs_c[tid + 1] - 1) / 4; ++q) g |= 1u << q;
     sm.pg_c[tid] = (uint16_t)g;
-    uint32_t r8 = 0, c8 = 0;
-    if (tid < t.natI && sm.rs_r[tid + 1] > sm.rs_r[tid])
-      for (int q = sm.rs_r[tid] / 8; q <= (sm.rs_r[tid + 1] - 1) / 8; ++q) r8 |= 1u << q;
-    if (tid < t.natJ && sm.rs_c[tid + 1] > sm.rs_c[tid])
-      for (int q = sm.rs_c[tid] / 8; q <= (sm.rs_c[tid + 1] - 1) / 8; ++q) c8 |= 1u << q;
-    sm.r8_r[tid] = (uint8_t)r8;
-    sm.r8_c[tid] = (uint8_t)c8;
+    if (tid < t.natI) atomicMax(&sm.max_rows, sm.rs_r[tid + 1] - sm.rs_r[tid]);
+    if (tid < t.natJ) atomicMax(&sm.max_cols, sm.rs_c[tid + 1] - sm.rs_c[tid]);
   }
   __syncthreads();
   return t;
@@ -240,32 +241,40 @@ __device__ __forceinline__ void tile_epilogue(
   }
   __syncthreads();
   // norms: one thread per block, one FMA chain over its values in
-  // column-major order; sixteen values are read ahead of their FMAs (a
-  // padding zero past the end leaves the sum unchanged: fma(0, 0, s) = s)
-  for (int pi = tid; pi < npairs; pi += NT) {
-    const int id = sm.pair_id[pi];
-    const int pr = id >> 4, pc = id & 15;
-    const int r0 = sm.rs_r[pr], m = sm.rs_r[pr + 1] - r0;
-    const int c0 = sm.rs_c[pc], n = sm.rs_c[pc + 1] - c0;
-    const int len = m * n;
-    double s = 0.0;
-    int r = 0, a = r0 * kPitch + c0;
-    for (int e0 = 0; e0 < len; e0 += 16) {
-      double x[16];
+  // column-major order. Every thread walks the tile's largest block shape,
+  // column by column in steps of eight rows; a slot outside its own block
+  // reads sm.zero, and fma(0, 0, s) = s leaves the chain unchanged. The
+  // eight reads of a step are issued before its FMAs. Waves without a block
+  // skip.
+  {
+    const int mmax = __builtin_amdgcn_readfirstlane(sm.max_rows);
+    const int nmax = __builtin_amdgcn_readfirstlane(sm.max_cols);
+    for (int pi0 = 0; pi0 < npairs; pi0 += NT) {
+      if (pi0 + 64 * wave >= npairs) continue;
+      const int pi = pi0 + tid;
+      int pr = 0, pc = 0, m = 0, n = 0, base = 0;
+      if (pi < npairs) {
+        const int id = sm.pair_id[pi];
+        pr = id >> 4;
+        pc = id & 15;
+        m = sm.rs_r[pr + 1] - sm.rs_r[pr];
+        n = sm.rs_c[pc + 1] - sm.rs_c[pc];
+        base = sm.rs_r[pr] * kPitch + sm.rs_c[pc];
+      }
+      double s = 0.0;
+      for (int cc = 0; cc < nmax; ++cc) {
+        const int mc = cc < n ? m : 0;  // rows of this column in this thread's block
+        const double* col = Ct + base + cc;
+        for (int r0 = 0; r0 < mmax; r0 += 8) {
+          double x[8];
 #pragma unroll
-      for (int u = 0; u < 16; ++u) {
-        x[u] = e0 + u < len ? Ct[a] : 0.0;
-        if (++r == m) {
-          r = 0;
-          a += 1 - (m - 1) * kPitch;
-        } else {
-          a += kPitch;
+          for (int u = 0; u < 8; ++u) x[u] = *(r0 + u < mc ? col + (r0 + u) * kPitch : &sm.zero);
+#pragma unroll
+          for (int u = 0; u < 8; ++u) s = __builtin_fma(x[u], x[u], s);
         }
       }
-#pragma unroll
-      for (int u = 0; u < 16; ++u) s = __builtin_fma(x[u], x[u], s);
+      if (pi < npairs && s >= eps2) atomicOr(&sm.keptw[pr], 1u << pc);
     }
-    if (s >= eps2) atomicOr(&sm.keptw[pr], 1u << pc);
   }
   __syncthreads();
   if (tid < kBandAtoms) kept_out[(int64_t)t.tile * kBandAtoms + tid] = (uint16_t)sm.keptw[tid];
