@@ -154,7 +154,17 @@ def line(metric, value, unit, ms, args, config, dtype="f64", **extra):
            "vs_baseline": None, "dtype": dtype, "data": "synthetic (seeded water-like atom box)",
            "config": config}
     out.update(extra)
+    out["library"] = library()
     return out
+
+
+def library():
+    """The HIP library this run used (path, hash, build record match)."""
+    from lamb_amd import _native
+    info = _native.library_info()
+    if "sha256" in info:
+        info["sha256"] = info["sha256"][:16]
+    return info
 
 
 def run_config3(args):
@@ -587,6 +597,7 @@ def main():
     for h in (dcopy, dc, db, da, dd):
         h.close()
     ctx.close()
+    result["library"] = library()
     print(json.dumps(result), flush=True)
 
 

@@ -152,6 +152,26 @@ def load() -> C.CDLL:
     return lib
 
 
+def library_info() -> dict:
+    """Which library this process uses: its path and hash, and whether it is
+    the one the build record describes and was built from the sources in
+    this tree (the record is written by lamb_amd.build next to the library)."""
+    import hashlib
+    import json
+    info = {"path": os.fspath(LIB_PATH), "loaded": _lib is not None}
+    if not LIB_PATH.exists():
+        return info
+    info["sha256"] = hashlib.sha256(LIB_PATH.read_bytes()).hexdigest()
+    record = LIB_PATH.with_name("liblamb_hip.build.json")
+    if record.exists():
+        from lamb_amd import build as B
+        rec = json.loads(record.read_text())
+        info["built_utc"] = rec.get("built_utc")
+        info["matches_build_record"] = rec.get("library_sha256") == info["sha256"]
+        info["built_from_these_sources"] = rec.get("sources_sha256") == B.sources_sha256()
+    return info
+
+
 def ptr(a, ctype):
     """Pointer to a contiguous numpy array (or None)."""
     if a is None:

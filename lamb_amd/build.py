@@ -2,6 +2,9 @@
 from __future__ import annotations
 
 import concurrent.futures as cf
+import datetime
+import hashlib
+import json
 import os
 import pathlib
 import subprocess
@@ -10,6 +13,7 @@ import sys
 HERE = pathlib.Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OUT = HERE / "liblamb_hip.so"
+RECORD = HERE / "liblamb_hip.build.json"
 BUILD = HERE / "_build"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -43,9 +47,31 @@ def build(verbose: bool = False) -> pathlib.Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, OUT)
+    RECORD.write_text(json.dumps({
+        "library": OUT.name,
+        "library_sha256": file_sha256(OUT),
+        "sources_sha256": sources_sha256(),
+        "built_utc": datetime.datetime.now(datetime.timezone.utc).isoformat(timespec="seconds"),
+        "hipcc": HIPCC,
+        "flags": FLAGS,
+    }, indent=1) + "\n")
     if verbose:
         print(f"built {OUT}")
     return OUT
+
+
+def file_sha256(path: pathlib.Path) -> str:
+    return hashlib.sha256(path.read_bytes()).hexdigest()
+
+
+def sources_sha256() -> str:
+    """One hash over the HIP sources and the public header, in name order."""
+    h = hashlib.sha256()
+    for f in sorted(CSRC.glob("*")) + [HERE.parent / "include" / "lamb_hip.h"]:
+        if f.is_file():
+            h.update(f.name.encode())
+            h.update(f.read_bytes())
+    return h.hexdigest()
 
 
 if __name__ == "__main__":
