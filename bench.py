@@ -25,9 +25,9 @@ PEAK_HBM_GBS = 8000.0
 # from profiles/numeric_f64_half_kernel_water6144.md. A kernel or order
 # without a profile reports null.
 PROFILED_TRAFFIC_BYTES = {
-    ("numeric_f64_half_kernel", 4): 104.58e9,
-    ("numeric_f64_half_kernel", 8): 103.65e9,
-    ("numeric_f64_half_kernel", 16): 117.48e9,
+    ("numeric_f64_half_kernel", 4): 104.85e9,
+    ("numeric_f64_half_kernel", 8): 104.04e9,
+    ("numeric_f64_half_kernel", 16): 117.92e9,
 }
 # FP64 numeric kernel of each variant (lamb_hip_tuning_info)
 F64_KERNELS = {0: "numeric_f64_kernel", 1: "numeric_f64_ksplit_kernel",

@@ -165,7 +165,11 @@ int32_t lamb_hip_matrix_info(const lamb_hip_mat* mat, int64_t* n_blks, int64_t* 
  * Triple (i,k,j) is kept iff nsq(A_ik)*nsq(B_kj) >= eps_filter^2; C_ij is
  * stored iff nsq(C_ij) >= eps_filter^2, where nsq is the sum of squares of a
  * block's elements in column-major order, one fused multiply-add per step.
- * beta == 0 ignores C's old contents. C may not alias A or B. */
+ * beta == 0 ignores C's old contents. C may not alias A or B. Products of
+ * a distribution with an atom of more than 64 rows (the reference's int8
+ * block size allows up to 126) are refused with LAMB_HIP_UNIMPLEMENTED:
+ * the MFMA tiles hold at most 64 rows of one atom. Storage, exchange and the
+ * group functions accept such atoms. */
 int32_t lamb_hip_multiply(lamb_hip_ctx* ctx, const lamb_hip_mat* A, const lamb_hip_mat* B,
                           lamb_hip_mat* C, double alpha, double beta, double eps_filter,
                           int32_t flags, lamb_hip_stats* stats);

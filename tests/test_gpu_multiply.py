@@ -230,6 +230,36 @@ def test_upload_download_round_trip(ctx):
     dd.close()
 
 
+def test_atoms_above_64_rows_stored_but_not_multiplied(ctx):
+    """Atoms of 70 and 126 rows (the reference's block size allows up to 126):
+    upload, download and trace work; products are refused as unimplemented
+    (the MFMA tiles hold at most 64 rows of one atom), with the library left
+    usable."""
+    base = S.water_system(60, seed=95)
+    blk = base.blk.copy()
+    blk[3], blk[17] = 70, 126
+    atoms = S.AtomBox(base.pos, blk, base.box)
+    a = S.decay_matrix(atoms, 8.0, 0.4, 96)
+    print(f"case: atoms of {sorted(set(blk.tolist()))} rows, {a.n_blocks} blocks", flush=True)
+    dist = Distribution(atoms.blk, atoms.pos, atoms.box)
+    lm = LambMatrix.from_global(dist, a)
+    dd = D.DeviceDistribution(ctx, dist)
+    dm = D.DeviceMatrix.from_lamb(dd, lm)
+    back = dm.download()
+    for s0, s1 in zip(lm.sectors, back.sectors):
+        assert np.array_equal(s0.row_p, s1.row_p) and np.array_equal(s0.col_i, s1.col_i)
+        assert np.array_equal(s0.extents, s1.extents) and np.array_equal(s0.data, s1.data)
+    assert dm.trace() == pytest.approx(oracle.trace(a), rel=1e-14, abs=1e-14)
+    dc = D.DeviceMatrix(dd)
+    with pytest.raises(D.LambError) as err:
+        D.multiply(dm, dm, dc, eps_filter=1e-7)
+    assert err.value.type == -1000  # unimplemented
+    assert "block dimension 126 above 64" in str(err.value)
+    for h in (dc, dm, dd):
+        h.close()
+    test_tiny_mixed(ctx)  # the context still multiplies
+
+
 def test_default_kernel_is_deterministic_and_variants_agree(monkeypatch):
     """The FP64 variant is chosen by rule (half kernel on this sparse box) and
     the autotuner only times tile orders, which permute the tile-to-workgroup
