@@ -213,6 +213,37 @@ def test_zero_size_blocks(ctx):
     check(ctx, atoms, a, b, eps=1e-3)
 
 
+def random_fill(blk, fill, seed):
+    """Block matrix with each block (i, j) stored with probability fill and
+    U[-1, 1] values: the shape of the reference's lamb_matrix_X_test matrices
+    (src/lamb_matrix_X_tests.F90:102,128,202: up to 100 atoms, block sizes
+    0..3, 70 % of blocks)."""
+    rng = np.random.default_rng(seed)
+    n = blk.size
+    keep = rng.random((n, n)) < fill
+    row_p = np.concatenate([[0], np.cumsum(keep.sum(axis=1))]).astype(np.int64)
+    rows, col = np.nonzero(keep)
+    sizes = blk[rows].astype(np.int64) * blk[col]
+    off = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    return S.BlockCSR(blk.copy(), row_p, col.astype(np.int32), off,
+                      rng.uniform(-1.0, 1.0, int(sizes.sum())))
+
+
+@pytest.mark.parametrize("n_procs", [1, 3])
+def test_reference_test_shapes(ctx, n_procs):
+    rng = np.random.default_rng(97 + n_procs)
+    n = 100
+    blk = rng.integers(0, 4, n).astype(np.int32)
+    box = float((n / 0.1) ** (1.0 / 3.0))
+    atoms = S.AtomBox(S._wrap32(rng.uniform(0.0, box, (n, 3)), box), blk, box)
+    a = random_fill(blk, 0.7, 98)
+    b = random_fill(blk, 0.7, 99)
+    print(f"case: {n} atoms of sizes 0..3, 70 % fill, {n_procs} procs: A {a.n_blocks} "
+          f"blocks", flush=True)
+    for eps in (0.0, 0.3):
+        check(ctx, atoms, a, b, eps=eps, n_procs=n_procs)
+
+
 def test_upload_download_round_trip(ctx):
     atoms = S.water_system(200, seed=91, heavy_fraction=0.1)
     a = S.decay_matrix(atoms, 8.0, 0.4, 92)
