@@ -11,18 +11,7 @@ __global__ void fill_i64(int64_t* __restrict__ p, int64_t n, int64_t v);
 __global__ void copy_i64(int64_t* __restrict__ dst, const int64_t* __restrict__ src, int64_t n);
 
 // wave-wide helpers (wave64)
-__device__ __forceinline__ uint32_t wave_uniform(uint32_t v) {
-  return __builtin_amdgcn_readfirstlane(v);
-}
-__device__ __forceinline__ int wave_min_i32(int v) {
-  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-  return v;
-}
 __device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-__device__ __forceinline__ int wave_sum_i32(int v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
@@ -35,11 +24,6 @@ __device__ __forceinline__ int64_t wave_excl_i64(int64_t v) {
     if (lane >= o) x += y;
   }
   return x - v;
-}
-// number of set lanes below this lane in a 64-bit ballot
-__device__ __forceinline__ int lanes_below(uint64_t ballot) {
-  return __builtin_amdgcn_mbcnt_hi((uint32_t)(ballot >> 32),
-                                   __builtin_amdgcn_mbcnt_lo((uint32_t)ballot, 0));
 }
 
 // exclusive scan of in[0..n) into out[0..n], out[n] = total (scan.hip)

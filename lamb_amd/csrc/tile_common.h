@@ -28,11 +28,6 @@ struct TileInfo {
   const TileDesc* tp;
 };
 
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
 // Tile of this workgroup and its band geometry in LDS. Workgroup b runs
 // perm[b]. The hardware deals consecutive workgroups round-robin over the 8
 // XCDs, so the tiles running at any moment are a window of the tile order
