@@ -19,6 +19,8 @@ struct TileSmem {
   uint32_t keptw[kBandAtoms];
   int32_t max_rows, max_cols;  // largest row / column atom of the tile
   double zero;                 // read by norm-chain slots outside a block
+  int32_t n_exact;             // blocks whose post-filter needs the exact chain
+  int16_t exact_id[kBandAtoms * kBandAtoms];
 };
 
 struct TileInfo {
@@ -62,6 +64,7 @@ __device__ __forceinline__ TileInfo tile_setup(const TileDesc* __restrict__ tile
     sm.max_rows = 0;
     sm.max_cols = 0;
     sm.zero = 0.0;
+    sm.n_exact = 0;
   }
   __syncthreads();
   if (tid < kBandAtoms) {
@@ -217,6 +220,7 @@ __device__ __forceinline__ void tile_epilogue(
     // without a division per element
     const int q64 = 64 / m, r64 = 64 - q64 * m;
     int cc = lane / m, r = lane - (lane / m) * m;
+    double part = 0.0;  // this lane's squares, one FMA each
     for (int e = lane; e < len; e += 64) {
       double* x = &Ct[(sm.rs_r[pr] + r) * kPitch + sm.rs_c[pc] + cc];
       double v = alpha * *x;
@@ -226,6 +230,7 @@ __device__ __forceinline__ void tile_epilogue(
       }
       c_data[base + e] = v;
       *x = v;
+      part = __builtin_fma(v, v, part);
       r += r64;
       cc += q64;
       if (r >= m) {
@@ -233,23 +238,42 @@ __device__ __forceinline__ void tile_epilogue(
         cc += 1;
       }
     }
+    // Post-filter decision from a cheap sum when it is certain. Every term
+    // is a square, so both this sum (lane partials of at most ceil(len/64)
+    // FMAs, then six butterfly additions) and the contract's sequential FMA
+    // chain over the len values are within gamma_k * S of the exact sum S,
+    // gamma_k = k u / (1 - k u), u = 2^-53. Their difference is below
+    // 2 (len + ceil(len/64) + 8) u * sum; when the sum lies farther than
+    // that from eps2, the chain is on the same side of eps2, and the keep
+    // bit is exactly the chain's. Other blocks (near eps2, a sum that is
+    // tiny, zero, infinite or NaN) get the chain itself below.
+    double sum = part;
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+    sum = __shfl(sum, 0, 64);  // every lane takes lane 0's association
+    const double margin = sum * ((double)(len + (len + 63) / 64 + 8) * 0x1p-52);
+    const bool certain = sum > 0x1p-930 && sum <= 0x1p+1000 && fabs(sum - eps2) > margin;
+    if (lane == 0) {
+      if (!certain) sm.exact_id[atomicAdd(&sm.n_exact, 1)] = (int16_t)pi;
+      else if (sum >= eps2) atomicOr(&sm.keptw[pr], 1u << pc);
+    }
   }
   __syncthreads();
-  // norms: one thread per block, one FMA chain over its values in
-  // column-major order. Every thread walks the tile's largest block shape,
-  // column by column in steps of eight rows; a slot outside its own block
-  // reads sm.zero, and fma(0, 0, s) = s leaves the chain unchanged. The
-  // eight reads of a step are issued before its FMAs. Waves without a block
-  // skip.
-  {
+  // norms of the undecided blocks: one thread per block, one FMA chain over
+  // its values in column-major order (the contract's order). Every thread
+  // walks the tile's largest block shape, column by column in steps of
+  // eight rows; a slot outside its own block reads sm.zero, and
+  // fma(0, 0, s) = s leaves the chain unchanged. The eight reads of a step
+  // are issued before its FMAs. Waves without a block skip.
+  const int n_exact = __builtin_amdgcn_readfirstlane(sm.n_exact);
+  if (n_exact > 0) {
     const int mmax = __builtin_amdgcn_readfirstlane(sm.max_rows);
     const int nmax = __builtin_amdgcn_readfirstlane(sm.max_cols);
-    for (int pi0 = 0; pi0 < npairs; pi0 += NT) {
-      if (pi0 + 64 * wave >= npairs) continue;
+    for (int pi0 = 0; pi0 < n_exact; pi0 += NT) {
+      if (pi0 + 64 * wave >= n_exact) continue;
       const int pi = pi0 + tid;
       int pr = 0, pc = 0, m = 0, n = 0, base = 0;
-      if (pi < npairs) {
-        const int id = sm.pair_id[pi];
+      if (pi < n_exact) {
+        const int id = sm.pair_id[sm.exact_id[pi]];
         pr = id >> 4;
         pc = id & 15;
         m = sm.rs_r[pr + 1] - sm.rs_r[pr];
@@ -268,7 +292,7 @@ __device__ __forceinline__ void tile_epilogue(
           for (int u = 0; u < 8; ++u) s = __builtin_fma(x[u], x[u], s);
         }
       }
-      if (pi < npairs && s >= eps2) atomicOr(&sm.keptw[pr], 1u << pc);
+      if (pi < n_exact && s >= eps2) atomicOr(&sm.keptw[pr], 1u << pc);
     }
   }
   __syncthreads();
