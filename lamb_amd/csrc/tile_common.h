@@ -23,6 +23,34 @@ struct TileSmem {
   int16_t exact_id[kBandAtoms * kBandAtoms];
 };
 
+// v of the lane a DPP control selects (0 where it selects none), for a
+// 64-bit value as two 32-bit moves
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, kCtrl, kRowMask,
+                                                           0xf, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), kCtrl,
+                                                           kRowMask, 0xf, false);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Sum over the wave's 64 lanes, returned to every lane: row prefix sums by
+// row_shr 1, 2, 4, 8, then row_bcast 15 and 31 carry the row totals into
+// lane 63 (each value meets at most 6 additions).
+__device__ __forceinline__ double wave_total_dpp(double v) {
+  v += dpp_f64<0x111, 0xf>(v);
+  v += dpp_f64<0x112, 0xf>(v);
+  v += dpp_f64<0x114, 0xf>(v);
+  v += dpp_f64<0x118, 0xf>(v);
+  v += dpp_f64<0x142, 0xa>(v);
+  v += dpp_f64<0x143, 0xc>(v);
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, 63);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), 63);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
 struct TileInfo {
   int tile, I, J, e0, ne, ktot, natI, natJ;
   int32_t firstI, firstJ;
@@ -240,16 +268,14 @@ __device__ __forceinline__ void tile_epilogue(
     }
     // Post-filter decision from a cheap sum when it is certain. Every term
     // is a square, so both this sum (lane partials of at most ceil(len/64)
-    // FMAs, then six butterfly additions) and the contract's sequential FMA
+    // FMAs, then at most six additions) and the contract's sequential FMA
     // chain over the len values are within gamma_k * S of the exact sum S,
     // gamma_k = k u / (1 - k u), u = 2^-53. Their difference is below
     // 2 (len + ceil(len/64) + 8) u * sum; when the sum lies farther than
     // that from eps2, the chain is on the same side of eps2, and the keep
     // bit is exactly the chain's. Other blocks (near eps2, a sum that is
     // tiny, zero, infinite or NaN) get the chain itself below.
-    double sum = part;
-    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
-    sum = __shfl(sum, 0, 64);  // every lane takes lane 0's association
+    const double sum = wave_total_dpp(part);
     const double margin = sum * ((double)(len + (len + 63) / 64 + 8) * 0x1p-52);
     const bool certain = sum > 0x1p-930 && sum <= 0x1p+1000 && fabs(sum - eps2) > margin;
     if (lane == 0) {
