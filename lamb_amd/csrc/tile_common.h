@@ -35,20 +35,23 @@ __device__ __forceinline__ double dpp_f64(double v) {
   return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
-// Sum over the wave's 64 lanes, returned to every lane: row prefix sums by
-// row_shr 1, 2, 4, 8, then row_bcast 15 and 31 carry the row totals into
-// lane 63 (each value meets at most 6 additions).
-__device__ __forceinline__ double wave_total_dpp(double v) {
+// Sums over each 32-lane half of the wave, returned to the lanes of that
+// half: row prefix sums by row_shr 1, 2, 4, 8, then row_bcast 15 carries
+// row 0's total into row 1 and row 2's into row 3 (each value meets at most
+// five additions); lanes 31 and 63 hold the two half totals.
+__device__ __forceinline__ double half_wave_totals_dpp(double v, int half) {
   v += dpp_f64<0x111, 0xf>(v);
   v += dpp_f64<0x112, 0xf>(v);
   v += dpp_f64<0x114, 0xf>(v);
   v += dpp_f64<0x118, 0xf>(v);
   v += dpp_f64<0x142, 0xa>(v);
-  v += dpp_f64<0x143, 0xc>(v);
   const uint64_t b = __builtin_bit_cast(uint64_t, v);
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, 63);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), 63);
-  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+  const uint32_t lo0 = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, 31);
+  const uint32_t hi0 = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), 31);
+  const uint32_t lo1 = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, 63);
+  const uint32_t hi1 = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), 63);
+  const uint64_t r = half ? (((uint64_t)hi1 << 32) | lo1) : (((uint64_t)hi0 << 32) | lo0);
+  return __builtin_bit_cast(double, r);
 }
 
 struct TileInfo {
@@ -219,16 +222,23 @@ __device__ __forceinline__ void tile_epilogue(
   }
   const int npairs = cbase;
 
-  // values: one wave per block, lanes over its elements
-  for (int pi = wave; pi < npairs; pi += kWaves) {
-    const int id = sm.pair_id[pi];
-    const int pr = id >> 4, pc = id & 15;
-    const int m = sm.rs_r[pr + 1] - sm.rs_r[pr];
-    const int n = sm.rs_c[pc + 1] - sm.rs_c[pc];
-    const int len = m * n;
-    const int64_t base = t.out_base + sm.pair_off[pi];
+  // values: one half-wave (32 lanes) per block, two blocks per wave at a
+  // time, lanes over the block's elements
+  const int half = lane >> 5, hl = lane & 31;
+  for (int pi0 = wave; pi0 < npairs; pi0 += 2 * kWaves) {
+    const int pi = pi0 + half * kWaves;
+    int pr = 0, pc = 0, m = 1, len = 0;
+    int64_t base = 0;
+    if (pi < npairs) {
+      const int id = sm.pair_id[pi];
+      pr = id >> 4;
+      pc = id & 15;
+      m = sm.rs_r[pr + 1] - sm.rs_r[pr];
+      len = m * (sm.rs_c[pc + 1] - sm.rs_c[pc]);
+      base = t.out_base + sm.pair_off[pi];
+    }
     const double* old = nullptr;
-    if (beta != 0.0 && old_row_p != nullptr) {
+    if (beta != 0.0 && old_row_p != nullptr && pi < npairs) {
       const int32_t gi = t.firstI + pr, gj = t.firstJ + pc;
       int64_t lo = old_row_p[gi], hi = old_row_p[gi + 1] - 1;
       int64_t found = -1;
@@ -244,12 +254,13 @@ __device__ __forceinline__ void tile_epilogue(
       }
       if (found >= 0) old = old_data + old_off[found];
     }
-    // element e = r + m * cc of the column-major block, stepped by 64
-    // without a division per element
-    const int q64 = 64 / m, r64 = 64 - q64 * m;
-    int cc = lane / m, r = lane - (lane / m) * m;
+    // element e = r + m * cc of the column-major block, stepped by 32
+    // without a division per element (a zero-row atom has len 0)
+    const int mm = m > 0 ? m : 1;
+    const int q32 = 32 / mm, r32 = 32 - q32 * mm;
+    int cc = hl / mm, r = hl - (hl / mm) * mm;
     double part = 0.0;  // this lane's squares, one FMA each
-    for (int e = lane; e < len; e += 64) {
+    for (int e = hl; e < len; e += 32) {
       double* x = &Ct[(sm.rs_r[pr] + r) * kPitch + sm.rs_c[pc] + cc];
       double v = alpha * *x;
       if (old) {
@@ -259,26 +270,26 @@ __device__ __forceinline__ void tile_epilogue(
       c_data[base + e] = v;
       *x = v;
       part = __builtin_fma(v, v, part);
-      r += r64;
-      cc += q64;
-      if (r >= m) {
-        r -= m;
+      r += r32;
+      cc += q32;
+      if (r >= mm) {
+        r -= mm;
         cc += 1;
       }
     }
     // Post-filter decision from a cheap sum when it is certain. Every term
-    // is a square, so both this sum (lane partials of at most ceil(len/64)
-    // FMAs, then at most six additions) and the contract's sequential FMA
+    // is a square, so both this sum (lane partials of at most ceil(len/32)
+    // FMAs, then at most five additions) and the contract's sequential FMA
     // chain over the len values are within gamma_k * S of the exact sum S,
     // gamma_k = k u / (1 - k u), u = 2^-53. Their difference is below
-    // 2 (len + ceil(len/64) + 8) u * sum; when the sum lies farther than
+    // 2 (len + ceil(len/32) + 8) u * sum; when the sum lies farther than
     // that from eps2, the chain is on the same side of eps2, and the keep
     // bit is exactly the chain's. Other blocks (near eps2, a sum that is
     // tiny, zero, infinite or NaN) get the chain itself below.
-    const double sum = wave_total_dpp(part);
-    const double margin = sum * ((double)(len + (len + 63) / 64 + 8) * 0x1p-52);
+    const double sum = half_wave_totals_dpp(part, half);
+    const double margin = sum * ((double)(len + (len + 31) / 32 + 8) * 0x1p-52);
     const bool certain = sum > 0x1p-930 && sum <= 0x1p+1000 && fabs(sum - eps2) > margin;
-    if (lane == 0) {
+    if (hl == 0 && pi < npairs) {
       if (!certain) sm.exact_id[atomicAdd(&sm.n_exact, 1)] = (int16_t)pi;
       else if (sum >= eps2) atomicOr(&sm.keptw[pr], 1u << pc);
     }

@@ -1,8 +1,11 @@
 // The post-filter shortcut of the tile epilogue: when |sum - eps2| > margin,
 // the contract's sequential FMA chain is on the same side of eps2 as the
-// 64-lane partial sum plus butterfly. Random blocks up to 64 x 64, eps2
-// placed 1e-17..1e-9 (relative) from the sum, or equal to the chain; exit 1
-// on any disagreement.
+// epilogue's cheap sum. That sum is formed as on the GPU: 32 lanes of a
+// half-wave, lane l one FMA per element e = l, l + 32, ..., then the DPP
+// tree (row prefix sums by row_shr 1, 2, 4, 8 within rows of 16 lanes, then
+// row 0's lane 15 added into row 1), total in lane 31. Random blocks up to
+// 64 x 64, eps2 placed 1e-17..1e-9 (relative) from the sum, or equal to the
+// chain; exit 1 on any disagreement.
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -17,22 +20,21 @@ int main(void) {
     for (int e = 0; e < len; ++e) x[e] = (2 * rnd() - 1) * scale * (rnd() < 0.1 ? 1e-8 : 1.0);
     double seq = 0.0;
     for (int e = 0; e < len; ++e) seq = fma(x[e], x[e], seq);
-    double part[64];
-    for (int l = 0; l < 64; ++l) {
-      part[l] = 0.0;
-      for (int e = l; e < len; e += 64) part[l] = fma(x[e], x[e], part[l]);
+    double v[32];
+    for (int l = 0; l < 32; ++l) {
+      v[l] = 0.0;
+      for (int e = l; e < len; e += 32) v[l] = fma(x[e], x[e], v[l]);
     }
-    // butterfly as lane 0 sees it: v0 += v(0^o) with every lane updated per level
-    for (int o = 32; o > 0; o >>= 1) {
-      double nv[64];
-      for (int l = 0; l < 64; ++l) nv[l] = part[l] + part[l ^ o];
-      for (int l = 0; l < 64; ++l) part[l] = nv[l];
+    for (int k = 1; k <= 8; k <<= 1) {  // row_shr k: lane l reads l - k in its row, else 0
+      double nv[32];
+      for (int l = 0; l < 32; ++l) nv[l] = v[l] + ((l % 16) >= k ? v[l - k] : 0.0);
+      for (int l = 0; l < 32; ++l) v[l] = nv[l];
     }
-    const double sum = part[0];
-    // eps2 very close to the sum: relative offsets from 1e-17 to 1e-9
+    for (int l = 16; l < 32; ++l) v[l] = v[l] + v[15];  // row_bcast:15 into row 1
+    const double sum = v[31];
     const double rel = ldexp(1.0, -(rand() % 30) - 27) * (rand() % 2 ? 1 : -1);
     const double eps2 = it % 4 == 0 ? seq : sum * (1.0 + rel);
-    const double margin = sum * ((double)(len + (len + 63) / 64 + 8) * 0x1p-52);
+    const double margin = sum * ((double)(len + (len + 31) / 32 + 8) * 0x1p-52);
     const int cert = sum > 0x1p-930 && sum <= 0x1p+1000 && fabs(sum - eps2) > margin;
     ++trials;
     if (cert) {
