@@ -35,22 +35,22 @@ __device__ __forceinline__ double dpp_f64(double v) {
   return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
-// Sums over each 32-lane half of the wave, returned to the lanes of that
-// half: row prefix sums by row_shr 1, 2, 4, 8, then row_bcast 15 carries
-// row 0's total into row 1 and row 2's into row 3 (each value meets at most
-// five additions); lanes 31 and 63 hold the two half totals.
-__device__ __forceinline__ double half_wave_totals_dpp(double v, int half) {
+// Sums over each 16-lane row of the wave, returned to the lanes of that row
+// (g = lane >> 4): row prefix sums by row_shr 1, 2, 4, 8 (each value meets
+// at most four additions) leave the row totals in lanes 15, 31, 47 and 63.
+__device__ __forceinline__ double row_totals_dpp(double v, int g) {
   v += dpp_f64<0x111, 0xf>(v);
   v += dpp_f64<0x112, 0xf>(v);
   v += dpp_f64<0x114, 0xf>(v);
   v += dpp_f64<0x118, 0xf>(v);
-  v += dpp_f64<0x142, 0xa>(v);
   const uint64_t b = __builtin_bit_cast(uint64_t, v);
-  const uint32_t lo0 = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, 31);
-  const uint32_t hi0 = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), 31);
-  const uint32_t lo1 = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, 63);
-  const uint32_t hi1 = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), 63);
-  const uint64_t r = half ? (((uint64_t)hi1 << 32) | lo1) : (((uint64_t)hi0 << 32) | lo0);
+  uint64_t r = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, 16 * q + 15);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), 16 * q + 15);
+    if (g == q) r = ((uint64_t)hi << 32) | lo;
+  }
   return __builtin_bit_cast(double, r);
 }
 
@@ -222,11 +222,11 @@ __device__ __forceinline__ void tile_epilogue(
   }
   const int npairs = cbase;
 
-  // values: one half-wave (32 lanes) per block, two blocks per wave at a
-  // time, lanes over the block's elements
-  const int half = lane >> 5, hl = lane & 31;
-  for (int pi0 = wave; pi0 < npairs; pi0 += 2 * kWaves) {
-    const int pi = pi0 + half * kWaves;
+  // values: one 16-lane row of the wave per block, four blocks per wave at
+  // a time, lanes over the block's elements
+  const int grp = lane >> 4, gl = lane & 15;
+  for (int pi0 = wave; pi0 < npairs; pi0 += 4 * kWaves) {
+    const int pi = pi0 + grp * kWaves;
     int pr = 0, pc = 0, m = 1, len = 0;
     int64_t base = 0;
     if (pi < npairs) {
@@ -254,13 +254,13 @@ __device__ __forceinline__ void tile_epilogue(
       }
       if (found >= 0) old = old_data + old_off[found];
     }
-    // element e = r + m * cc of the column-major block, stepped by 32
+    // element e = r + m * cc of the column-major block, stepped by 16
     // without a division per element (a zero-row atom has len 0)
     const int mm = m > 0 ? m : 1;
-    const int q32 = 32 / mm, r32 = 32 - q32 * mm;
-    int cc = hl / mm, r = hl - (hl / mm) * mm;
+    const int q16 = 16 / mm, r16 = 16 - q16 * mm;
+    int cc = gl / mm, r = gl - (gl / mm) * mm;
     double part = 0.0;  // this lane's squares, one FMA each
-    for (int e = hl; e < len; e += 32) {
+    for (int e = gl; e < len; e += 16) {
       double* x = &Ct[(sm.rs_r[pr] + r) * kPitch + sm.rs_c[pc] + cc];
       double v = alpha * *x;
       if (old) {
@@ -270,26 +270,26 @@ __device__ __forceinline__ void tile_epilogue(
       c_data[base + e] = v;
       *x = v;
       part = __builtin_fma(v, v, part);
-      r += r32;
-      cc += q32;
+      r += r16;
+      cc += q16;
       if (r >= mm) {
         r -= mm;
         cc += 1;
       }
     }
     // Post-filter decision from a cheap sum when it is certain. Every term
-    // is a square, so both this sum (lane partials of at most ceil(len/32)
-    // FMAs, then at most five additions) and the contract's sequential FMA
+    // is a square, so both this sum (lane partials of at most ceil(len/16)
+    // FMAs, then at most four additions) and the contract's sequential FMA
     // chain over the len values are within gamma_k * S of the exact sum S,
     // gamma_k = k u / (1 - k u), u = 2^-53. Their difference is below
-    // 2 (len + ceil(len/32) + 8) u * sum; when the sum lies farther than
+    // 2 (len + ceil(len/16) + 8) u * sum; when the sum lies farther than
     // that from eps2, the chain is on the same side of eps2, and the keep
     // bit is exactly the chain's. Other blocks (near eps2, a sum that is
     // tiny, zero, infinite or NaN) get the chain itself below.
-    const double sum = half_wave_totals_dpp(part, half);
-    const double margin = sum * ((double)(len + (len + 31) / 32 + 8) * 0x1p-52);
+    const double sum = row_totals_dpp(part, grp);
+    const double margin = sum * ((double)(len + (len + 15) / 16 + 8) * 0x1p-52);
     const bool certain = sum > 0x1p-930 && sum <= 0x1p+1000 && fabs(sum - eps2) > margin;
-    if (hl == 0 && pi < npairs) {
+    if (gl == 0 && pi < npairs) {
       if (!certain) sm.exact_id[atomicAdd(&sm.n_exact, 1)] = (int16_t)pi;
       else if (sum >= eps2) atomicOr(&sm.keptw[pr], 1u << pc);
     }

@@ -1,9 +1,8 @@
 // The post-filter shortcut of the tile epilogue: when |sum - eps2| > margin,
 // the contract's sequential FMA chain is on the same side of eps2 as the
-// epilogue's cheap sum. That sum is formed as on the GPU: 32 lanes of a
-// half-wave, lane l one FMA per element e = l, l + 32, ..., then the DPP
-// tree (row prefix sums by row_shr 1, 2, 4, 8 within rows of 16 lanes, then
-// row 0's lane 15 added into row 1), total in lane 31. Random blocks up to
+// epilogue's cheap sum. That sum is formed as on the GPU: the 16 lanes of a
+// wave row, lane l one FMA per element e = l, l + 16, ..., then the DPP row
+// prefix sums by row_shr 1, 2, 4, 8, total in lane 15. Random blocks up to
 // 64 x 64, eps2 placed 1e-17..1e-9 (relative) from the sum, or equal to the
 // chain; exit 1 on any disagreement.
 #include <math.h>
@@ -20,21 +19,20 @@ int main(void) {
     for (int e = 0; e < len; ++e) x[e] = (2 * rnd() - 1) * scale * (rnd() < 0.1 ? 1e-8 : 1.0);
     double seq = 0.0;
     for (int e = 0; e < len; ++e) seq = fma(x[e], x[e], seq);
-    double v[32];
-    for (int l = 0; l < 32; ++l) {
+    double v[16];
+    for (int l = 0; l < 16; ++l) {
       v[l] = 0.0;
-      for (int e = l; e < len; e += 32) v[l] = fma(x[e], x[e], v[l]);
+      for (int e = l; e < len; e += 16) v[l] = fma(x[e], x[e], v[l]);
     }
-    for (int k = 1; k <= 8; k <<= 1) {  // row_shr k: lane l reads l - k in its row, else 0
-      double nv[32];
-      for (int l = 0; l < 32; ++l) nv[l] = v[l] + ((l % 16) >= k ? v[l - k] : 0.0);
-      for (int l = 0; l < 32; ++l) v[l] = nv[l];
+    for (int k = 1; k <= 8; k <<= 1) {  // row_shr k: lane l reads l - k, else 0
+      double nv[16];
+      for (int l = 0; l < 16; ++l) nv[l] = v[l] + (l >= k ? v[l - k] : 0.0);
+      for (int l = 0; l < 16; ++l) v[l] = nv[l];
     }
-    for (int l = 16; l < 32; ++l) v[l] = v[l] + v[15];  // row_bcast:15 into row 1
-    const double sum = v[31];
+    const double sum = v[15];
     const double rel = ldexp(1.0, -(rand() % 30) - 27) * (rand() % 2 ? 1 : -1);
     const double eps2 = it % 4 == 0 ? seq : sum * (1.0 + rel);
-    const double margin = sum * ((double)(len + (len + 31) / 32 + 8) * 0x1p-52);
+    const double margin = sum * ((double)(len + (len + 15) / 16 + 8) * 0x1p-52);
     const int cert = sum > 0x1p-930 && sum <= 0x1p+1000 && fabs(sum - eps2) > margin;
     ++trials;
     if (cert) {

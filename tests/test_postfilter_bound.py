@@ -1,6 +1,6 @@
 """The tile epilogue decides a C block's post-filter bit from a cheap sum
-(32 lane partials of one FMA per element, then a DPP reduction tree)
-whenever that sum lies farther than (len + ceil(len/32) + 8) * 2^-52
+(16 lane partials of one FMA per element, then DPP row prefix sums)
+whenever that sum lies farther than (len + ceil(len/16) + 8) * 2^-52
 (relative) from eps^2, and runs the contract's sequential FMA chain
 otherwise. Every term is a square, so both sums are within gamma_k * S of
 the exact sum; the margin is twice their first-order difference. This checks the rule on the CPU
@@ -12,7 +12,7 @@ import subprocess
 import pytest
 
 SRC = pathlib.Path(__file__).resolve().parent / "cpu_src" / "postfilter_bound.c"
-MARGIN = "const double margin = sum * ((double)(len + (len + 31) / 32 + 8) * 0x1p-52);"
+MARGIN = "const double margin = sum * ((double)(len + (len + 15) / 16 + 8) * 0x1p-52);"
 
 
 def build(tmp_path, text):
