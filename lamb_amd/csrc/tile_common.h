@@ -35,6 +35,18 @@ __device__ __forceinline__ double dpp_f64(double v) {
   return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
+// Inclusive prefix sum over the wave's 64 lanes: row prefix sums by
+// row_shr 1, 2, 4, 8, then row_bcast 15 and 31 carry the earlier rows in.
+__device__ __forceinline__ int wave_scan_dpp(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);
+  return v;
+}
+
 // Sums over each 16-lane row of the wave, returned to the lanes of that row
 // (g = lane >> 4): row prefix sums by row_shr 1, 2, 4, 8 (each value meets
 // at most four additions) leave the row totals in lanes 15, 31, 47 and 63.
@@ -189,16 +201,13 @@ __device__ __forceinline__ void tile_epilogue(
     const bool present = ip < t.natI && jp < t.natJ && ((t.tp->pattern[ip] >> jp) & 1);
     const int64_t sz =
         present ? (int64_t)(sm.rs_r[ip + 1] - sm.rs_r[ip]) * (sm.rs_c[jp + 1] - sm.rs_c[jp]) : 0;
-    int cnt = present ? 1 : 0;
-    int64_t csz = sz;
-    for (int o = 1; o < 64; o <<= 1) {
-      const int c2 = __shfl_up(cnt, o, 64);
-      const int64_t s2 = __shfl_up(csz, o, 64);
-      if (lane >= o) {
-        cnt += c2;
-        csz += s2;
-      }
-    }
+    // inclusive prefix counts by ballot, sizes by a DPP scan (a tile holds
+    // at most 64 x 64 values, so 32 bits suffice)
+    const uint64_t pres = __ballot(present);
+    const int cnt = (int)__builtin_amdgcn_mbcnt_hi(
+                        (uint32_t)(pres >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)pres, 0u)) +
+                    (present ? 1 : 0);
+    const int64_t csz = wave_scan_dpp((int)sz);
     if (lane == 63) {
       sm.wave_cnt[wave] = cnt;
       sm.wave_sz[wave] = csz;
