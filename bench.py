@@ -20,14 +20,15 @@ METRIC = "block-SpGEMM GFLOP/s & HBM GB/s: FP64 plain vs BF16x3 batched at ≤1e
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of an FP64 numeric kernel on this workload, by kernel
-# name and tile-order row block: rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x
-# 1.000 (scale factors from a device copy of known size in the same run),
-# from profiles/numeric_f64_half_kernel_water6144.md. A kernel or order
-# without a profile reports null.
+# name, tile-order row block and launch form (persistent grid or not):
+# rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scale factors from a
+# device copy of known size in the same run), from
+# profiles/numeric_f64_half_kernel_water6144.md. A kernel, order or launch
+# form without a profile reports null.
 PROFILED_TRAFFIC_BYTES = {
-    ("numeric_f64_half_kernel", 4): 104.85e9,
-    ("numeric_f64_half_kernel", 8): 104.04e9,
-    ("numeric_f64_half_kernel", 16): 117.92e9,
+    ("numeric_f64_half_kernel", 4, False): 104.85e9,
+    ("numeric_f64_half_kernel", 8, False): 104.04e9,
+    ("numeric_f64_half_kernel", 16, False): 117.92e9,
 }
 # FP64 numeric kernel of each variant (lamb_hip_tuning_info)
 F64_KERNELS = {0: "numeric_f64_kernel", 1: "numeric_f64_ksplit_kernel",
@@ -540,6 +541,7 @@ def main():
     variant, _ = D.tuning_info(ctx)
     kname = F64_KERNELS.get(variant, str(variant))
     rowblock = D.tuning_order(ctx)
+    persistent, workgroups, _ = D.tuning_launch(ctx)
     result = {
         "metric": METRIC,
         "value": flops / (ms_step * 1e-3) / 1e9,
@@ -579,8 +581,10 @@ def main():
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tf / PEAK_FP64_TFLOPS,
-            "traffic": PROFILED_TRAFFIC_BYTES.get((kname, rowblock)),
+            "traffic": PROFILED_TRAFFIC_BYTES.get((kname, rowblock, persistent)),
             "tile_order_rowblock": rowblock,
+            "persistent_grid": persistent,
+            "workgroups": workgroups,
             "algorithmic_bytes": bytes_alg,
         },
     }

@@ -218,14 +218,22 @@ int32_t lamb_hip_trace_noalloc(const lamb_hip_mat* A, double* dev_trace);
  * quadrant|half|ksplit forces one (the variants add K products in different
  * orders: results then differ at rounding level). n_classes: workload classes
  * tuned so far. The autotuner times the tile orders (row bands per block of
- * the order in which workgroups take tiles: 4, 8, 16) on the first product of
- * a class and keeps the fastest; the order only maps tiles to workgroups, so
- * every candidate writes the same bits. */
+ * the order in which workgroups take tiles: 4, 8, 16), each launched with one
+ * workgroup per tile and as a persistent grid (the device's resident
+ * workgroups, each looping over tiles), on the first product of a class and
+ * keeps the fastest; order and launch form only map tiles to workgroups, so
+ * every candidate writes the same bits. LAMB_HIP_TILE_ROWBLOCK=n and
+ * LAMB_HIP_PERSISTENT=0|1 force them. */
 int32_t lamb_hip_tuning_info(const lamb_hip_ctx* ctx, int32_t* last_variant,
                              int32_t* n_classes);
 /* Row bands per block of the tile order the context's last multiply used (the
  * autotuner's choice for its workload class, or LAMB_HIP_TILE_ROWBLOCK). */
 int32_t lamb_hip_tuning_order(const lamb_hip_ctx* ctx, int32_t* last_rowblock);
+/* Launch form of the context's last numeric kernel (no counterpart in the
+ * reference): persistent = 1 when it ran as a persistent grid, workgroups =
+ * the grid launched, tiles = the product's C tiles (-1 before any multiply). */
+int32_t lamb_hip_tuning_launch(const lamb_hip_ctx* ctx, int32_t* persistent,
+                               int32_t* workgroups, int32_t* tiles);
 
 /* --- sector ownership (plexus) -------------------------------------------
  * The maps of lamb_plexus_calculate with force_n_procs

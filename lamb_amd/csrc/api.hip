@@ -111,6 +111,7 @@ int32_t lamb_hip_finalize(lamb_hip_ctx* ctx) {
   ctx->d_bexp.release();
   ctx->d_aslice.release();
   ctx->d_bslice.release();
+  ctx->d_slot_counter.release();
   for (auto& ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

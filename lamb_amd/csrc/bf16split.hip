@@ -28,6 +28,8 @@
 // at most 256 K columns every accumulator is added into FP64 and cleared,
 // so the cross-chain (cross-k) accumulation into C is FP64. Norms, filters
 // and the epilogue are FP64 and shared with the FP64 kernel.
+#include <algorithm>
+
 #include "common.h"
 #include "tile_common.h"
 
@@ -178,16 +180,16 @@ __global__ __launch_bounds__(256) void pack_slices_kernel(
 // readlane); masked lanes load the zero chunk so every step issues a fixed
 // number of loads.
 // ---------------------------------------------------------------------------
-template <int S>
-__global__ __launch_bounds__(256, 2) void numeric_bf16_kernel(
-    const TileDesc* __restrict__ tiles, const int32_t* __restrict__ perm,
+template <int S, bool kLoop>
+__device__ __forceinline__ void numeric_bf16_kernel_tile(
+    int32_t slot, const TileDesc* __restrict__ tiles, const int32_t* __restrict__ perm,
     const Entry* __restrict__ entries, const Band* __restrict__ bands,
     const uint4* __restrict__ a_sl, const uint4* __restrict__ b_sl, uint32_t a_zero,
     uint32_t b_zero, double alpha, double beta, double eps2,
     const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
     const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
-    double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
-  LAMB_GUARD;
+    double* __restrict__ c_data, uint16_t* __restrict__ kept_out,
+    int32_t* __restrict__ slot_counter, int32_t* next_slot) {
   __shared__ double lds[kBandRows * kPitch];
   uint2* tbl_ab = reinterpret_cast<uint2*>(lds);  // A, B chunk
   uint32_t* tbl_m = reinterpret_cast<uint32_t*>(tbl_ab + kTableChunks);  // rmask | cmask << 16
@@ -200,11 +202,11 @@ __global__ __launch_bounds__(256, 2) void numeric_bf16_kernel(
   // wave quadrants are disjoint), read by the epilogue
   __shared__ double ctile[kBandRows * kPitch];
 
-  const int tid = threadIdx.x;
+  const int tid = tile_tid<kLoop>();
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wr = wave >> 1, wc = wave & 1;
-  const TileInfo T = tile_setup(tiles, perm, bands, sm);
+  const TileInfo T = tile_setup(tiles, perm, bands, sm, slot, tid);
   uint32_t rbit[2], cbit[2];
   lane_atom_bits(T, sm, wr, wc, lane, rbit, cbit);
   const uint32_t gsh_r = 8 * wr, gsh_c = 8 * wc;
@@ -350,7 +352,39 @@ __global__ __launch_bounds__(256, 2) void numeric_bf16_kernel(
   flush();
   __syncthreads();
   tile_epilogue<256>(ctile, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data,
-                     c_data, kept_out);
+                     c_data, kept_out, tid, slot_counter,
+                     next_slot);
+}
+
+template <int S, bool kLoop>
+__global__ __launch_bounds__(256, 2) void numeric_bf16_kernel(
+    int32_t n_tiles, int32_t* __restrict__ slot_counter, const TileDesc* __restrict__ tiles,
+    const int32_t* __restrict__ perm,
+    const Entry* __restrict__ entries, const Band* __restrict__ bands,
+    const uint4* __restrict__ a_sl, const uint4* __restrict__ b_sl, uint32_t a_zero,
+    uint32_t b_zero, double alpha, double beta, double eps2,
+    const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
+    const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
+    double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
+  LAMB_GUARD;
+  if constexpr (kLoop) {
+    // slots: blockIdx.x first, then the counter's (set to the grid size
+    // before the launch), claimed during each tile's epilogue
+    __shared__ int32_t next_slot;
+    for (int32_t slot = blockIdx.x; slot < n_tiles;) {
+      numeric_bf16_kernel_tile<S, kLoop>(
+          slot, tiles, perm, entries, bands, a_sl, b_sl, a_zero, b_zero, alpha, beta, eps2,
+          old_row_p, old_col, old_off, old_data, c_data, kept_out, slot_counter,
+          &next_slot);
+      __syncthreads();  // next_slot is visible; the next set-up rewrites the LDS
+      slot = __builtin_amdgcn_readfirstlane(next_slot);
+    }
+  } else {
+    numeric_bf16_kernel_tile<S, kLoop>(
+        blockIdx.x, tiles, perm, entries, bands, a_sl, b_sl, a_zero, b_zero, alpha, beta, eps2,
+        old_row_p, old_col, old_off, old_data, c_data, kept_out, nullptr,
+        nullptr);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -393,22 +427,31 @@ hipError_t launch_numeric_bf16(hipStream_t s, int slices, int32_t n_tiles, const
                                uint32_t b_zero, double alpha, double beta, double eps2,
                                const int64_t* old_row_p, const int32_t* old_col,
                                const int64_t* old_off, const double* old_data, double* c_data,
-                               uint16_t* kept) {
+                               uint16_t* kept, bool persistent, int32_t* slot_counter,
+                               int32_t* grid_used) {
+  if (grid_used) *grid_used = 0;
   if (n_tiles == 0) return hipSuccess;
   const uint4* a = static_cast<const uint4*>(a_sl);
   const uint4* b = static_cast<const uint4*>(b_sl);
-  const dim3 grid((unsigned)n_tiles), block(256);
-#define LAMB_BF16_LAUNCH(SL)                                                                   \
-  hipLaunchKernelGGL(numeric_bf16_kernel<SL>, grid, block, 0, s, tiles, perm, entries, bands, a, \
-                     b, a_zero, b_zero, alpha, beta, eps2, old_row_p, old_col, old_off,         \
-                     old_data, c_data, kept)
+  auto go = [&](auto kernel) {
+    int32_t grid = n_tiles;
+    if (persistent) {
+      const int32_t r = resident_workgroups(reinterpret_cast<const void*>(kernel));
+      if (r > 0) grid = std::min(grid, r);
+      launch_slot_counter_reset(s, slot_counter, grid);
+    }
+    if (grid_used) *grid_used = grid;
+    hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(256), 0, s, n_tiles, slot_counter,
+                       tiles, perm, entries, bands, a, b, a_zero, b_zero, alpha, beta, eps2,
+                       old_row_p, old_col, old_off, old_data, c_data, kept);
+  };
+  // a persistent grid runs the looping instantiation
   switch (slices) {
-    case 3: LAMB_BF16_LAUNCH(3); break;
-    case 4: LAMB_BF16_LAUNCH(4); break;
-    case 5: LAMB_BF16_LAUNCH(5); break;
+    case 3: persistent ? go(numeric_bf16_kernel<3, true>) : go(numeric_bf16_kernel<3, false>); break;
+    case 4: persistent ? go(numeric_bf16_kernel<4, true>) : go(numeric_bf16_kernel<4, false>); break;
+    case 5: persistent ? go(numeric_bf16_kernel<5, true>) : go(numeric_bf16_kernel<5, false>); break;
     default: return hipErrorInvalidValue;
   }
-#undef LAMB_BF16_LAUNCH
   return hipGetLastError();
 }
 

@@ -110,7 +110,13 @@ hipError_t launch_block_nsq(hipStream_t s, int64_t nblk, const int32_t* d_rows,
 // FP64 numeric kernel variants: per-wave 32 x 32 quadrants, per-wave 64 x 32
 // halves with two waves splitting each half's K steps, or whole-tile waves
 // splitting the K steps four ways. whole: the product has no partial entry
-// (the half and K-split kernels then run without per-lane masks)
+// (the half and K-split kernels then run without per-lane masks). Every
+// numeric kernel runs tile slots blockIdx.x, blockIdx.x + gridDim.x, ... of
+// the tile order: persistent = false launches one workgroup per tile,
+// persistent = true the device's resident workgroups, each looping over its
+// slots (the first is blockIdx.x, later ones come from *slot_counter, a
+// device int32 the launch resets); both compute every tile the same way
+// (same bits). grid_used gets the grid launched.
 constexpr int kF64Quadrant = 0;
 constexpr int kF64Ksplit = 1;
 constexpr int kF64Half = 2;
@@ -120,7 +126,13 @@ hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* ti
                               uint32_t b_zero, double alpha, double beta, double eps2,
                               const int64_t* old_row_p, const int32_t* old_col,
                               const int64_t* old_off, const double* old_data, double* c_data,
-                              uint16_t* kept, int variant, bool whole);
+                              uint16_t* kept, int variant, bool whole, bool persistent,
+                              int32_t* slot_counter, int32_t* grid_used);
+// Enqueues *counter = first (the slot counter of a persistent grid).
+void launch_slot_counter_reset(hipStream_t s, int32_t* counter, int32_t first);
+// Workgroups of a 256-thread kernel resident on the current device at once
+// (the grid of a persistent launch; 0 when unknown).
+int32_t resident_workgroups(const void* kernel);
 // BF16-split numeric phase (bf16split.hip). ex: int32 per matrix row (A,
 // row_side) or column (B); slice packs: slices x (n_chunks + 1) x 64 x 16 B.
 hipError_t launch_slice_exponents(hipStream_t s, int64_t nblk, const int32_t* rows,
@@ -137,7 +149,8 @@ hipError_t launch_numeric_bf16(hipStream_t s, int slices, int32_t n_tiles, const
                                uint32_t b_zero, double alpha, double beta, double eps2,
                                const int64_t* old_row_p, const int32_t* old_col,
                                const int64_t* old_off, const double* old_data, double* c_data,
-                               uint16_t* kept);
+                               uint16_t* kept, bool persistent, int32_t* slot_counter,
+                               int32_t* grid_used);
 hipError_t launch_trace(hipStream_t s, int32_t n_atoms, const int64_t* row_p,
                         const int32_t* col, const int64_t* off, const int32_t* blk,
                         const double* data, double* out);

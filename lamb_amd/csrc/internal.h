@@ -82,11 +82,20 @@ struct lamb_hip_ctx {
   lamb::DevBuf d_kept, d_apack, d_bpack;
   lamb::DevBuf d_aexp, d_bexp;  // BF16 split: exponent per A row / B column
   lamb::DevBuf d_aslice, d_bslice;  // BF16 split: slice packs
+  lamb::DevBuf d_slot_counter;      // persistent numeric grids: next tile slot
   void* sym = nullptr;  // lamb::SymState*
-  // autotuner: tile-order row-block size per workload class (multiply.hip)
-  std::map<std::array<int64_t, 4>, int32_t> tune;
+  // autotuner: tile order (row bands per block) and launch form (one
+  // workgroup per tile, or the resident workgroups looping over tiles) per
+  // workload class (multiply.hip)
+  struct TuneChoice {
+    int32_t rowblock = 8;
+    bool persistent = false;
+  };
+  std::map<std::array<int64_t, 4>, TuneChoice> tune;
   int32_t last_variant = -1;   // FP64 variant of the last multiply
   int32_t last_rowblock = -1;  // tile-order row bands per block of the last multiply
+  int32_t last_grid = -1;      // workgroups of the last numeric launch
+  int32_t last_tiles = -1;     // C tiles of the last multiply
   // Sizes the host reads back from the device (graph.hip). kSizesLive: read
   // them (stream sync); kSizesRecord: read and append to size_log;
   // kSizesReplay (during a graph capture): take them from size_log in order

@@ -1,5 +1,9 @@
 // HIP kernels for gfx950: block norms, FP64 numeric phase, trace.
+#include <algorithm>
+#include <map>
+#include <mutex>
 #include <type_traits>
+#include <utility>
 
 #include "common.h"
 #include "tile_common.h"
@@ -156,13 +160,14 @@ constexpr int kGroups = kStepCols / 4;
 // per entry writes its columns; *partial becomes 1 when an entry of the
 // window is partial (its atoms cut below its strips'). Returns the window's
 // step count; the table and *partial are visible to all threads on return.
+// tid = threadIdx.x.
 template <int KS>
 __device__ __forceinline__ int build_step_table(const TileInfo& T, const TileSmem& sm,
                                                 const Entry* __restrict__ entries, int Q0,
                                                 uint2* col_ab, uint32_t* col_m, uint32_t* step_g,
-                                                uint32_t a_zero, uint32_t b_zero, int* partial) {
+                                                uint32_t a_zero, uint32_t b_zero, int* partial,
+                                                int tid) {
   static_assert(KS == 4 || KS == 8, "K columns per step");
-  const int tid = threadIdx.x;
   const int ncols = min(kTableCols, T.ktot - Q0);
   const int nsteps = (ncols + KS - 1) / KS;
   __syncthreads();  // the previous window is done with the table
@@ -194,15 +199,16 @@ __device__ __forceinline__ int build_step_table(const TileInfo& T, const TileSme
   return nsteps;
 }
 
-__global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
-    const TileDesc* __restrict__ tiles, const int32_t* __restrict__ perm,
+template <bool kLoop>
+__device__ __forceinline__ void numeric_f64_kernel_tile(
+    int32_t slot, const TileDesc* __restrict__ tiles, const int32_t* __restrict__ perm,
     const Entry* __restrict__ entries, const Band* __restrict__ bands,
     const double* __restrict__ a_pack, const double* __restrict__ b_pack, uint32_t a_zero,
     uint32_t b_zero, double alpha, double beta, double eps2,
     const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
     const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
-    double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
-  LAMB_GUARD;
+    double* __restrict__ c_data, uint16_t* __restrict__ kept_out,
+    int32_t* __restrict__ slot_counter, int32_t* next_slot) {
   // main loop: the step table of a window of K columns; epilogue: C tile
   __shared__ double lds[kBandRows * kPitch];
   uint2* col_ab = reinterpret_cast<uint2*>(lds);  // A, B panel column
@@ -212,11 +218,11 @@ __global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
                     sizeof(double) * kBandRows * kPitch, "step table exceeds LDS buffer");
   __shared__ TileSmem sm;
 
-  const int tid = threadIdx.x;
+  const int tid = tile_tid<kLoop>();
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wr = wave >> 1, wc = wave & 1;
-  const TileInfo T = tile_setup(tiles, perm, bands, sm);
+  const TileInfo T = tile_setup(tiles, perm, bands, sm, slot, tid);
   const int e0 = T.e0, ne = T.ne, ktot = T.ktot;
   uint32_t rbit[2], cbit[2];
   lane_atom_bits(T, sm, wr, wc, lane, rbit, cbit);
@@ -323,7 +329,7 @@ __global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
   for (int Q0 = 0; Q0 < ktot; Q0 += kTableCols) {
     const int nsteps = build_step_table<kStepCols>(T, sm, entries, Q0, col_ab, col_m, step_g,
                                                    a_zero, b_zero,
-                                        &win_partial);
+                                        &win_partial, tid);
     if (win_partial)
       walk(nsteps, std::true_type{});
     else
@@ -346,7 +352,39 @@ __global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
       }
   __syncthreads();
   tile_epilogue<256>(Ct, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data,
-                     c_data, kept_out);
+                     c_data, kept_out, tid, slot_counter,
+                     next_slot);
+}
+
+template <bool kLoop>
+__global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
+    int32_t n_tiles, int32_t* __restrict__ slot_counter, const TileDesc* __restrict__ tiles,
+    const int32_t* __restrict__ perm,
+    const Entry* __restrict__ entries, const Band* __restrict__ bands,
+    const double* __restrict__ a_pack, const double* __restrict__ b_pack, uint32_t a_zero,
+    uint32_t b_zero, double alpha, double beta, double eps2,
+    const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
+    const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
+    double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
+  LAMB_GUARD;
+  if constexpr (kLoop) {
+    // slots: blockIdx.x first, then the counter's (set to the grid size
+    // before the launch), claimed during each tile's epilogue
+    __shared__ int32_t next_slot;
+    for (int32_t slot = blockIdx.x; slot < n_tiles;) {
+      numeric_f64_kernel_tile<kLoop>(
+          slot, tiles, perm, entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta,
+          eps2, old_row_p, old_col, old_off, old_data, c_data, kept_out, slot_counter,
+          &next_slot);
+      __syncthreads();  // next_slot is visible; the next set-up rewrites the LDS
+      slot = __builtin_amdgcn_readfirstlane(next_slot);
+    }
+  } else {
+    numeric_f64_kernel_tile<kLoop>(
+        blockIdx.x, tiles, perm, entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta,
+        eps2, old_row_p, old_col, old_off, old_data, c_data, kept_out, nullptr,
+        nullptr);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -381,16 +419,16 @@ __device__ __forceinline__ uint32_t tile_blocks(uint32_t g) {
 
 // kWhole: every entry of the product covers its strips' atoms (no
 // kEntryPartial), so lanes load whole panel columns without masks.
-template <bool kWhole>
-__global__ __launch_bounds__(256, 2) void numeric_f64_ksplit_kernel(
-    const TileDesc* __restrict__ tiles, const int32_t* __restrict__ perm,
+template <bool kWhole, bool kLoop>
+__device__ __forceinline__ void numeric_f64_ksplit_kernel_tile(
+    int32_t slot, const TileDesc* __restrict__ tiles, const int32_t* __restrict__ perm,
     const Entry* __restrict__ entries, const Band* __restrict__ bands,
     const double* __restrict__ a_pack, const double* __restrict__ b_pack, uint32_t a_zero,
     uint32_t b_zero, double alpha, double beta, double eps2,
     const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
     const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
-    double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
-  LAMB_GUARD;
+    double* __restrict__ c_data, uint16_t* __restrict__ kept_out,
+    int32_t* __restrict__ slot_counter, int32_t* next_slot) {
   __shared__ double lds[kBandRows * kPitch];
   uint2* col_ab = reinterpret_cast<uint2*>(lds);
   uint32_t* col_m = reinterpret_cast<uint32_t*>(col_ab + kTableCols);
@@ -398,10 +436,10 @@ __global__ __launch_bounds__(256, 2) void numeric_f64_ksplit_kernel(
   __shared__ TileSmem sm;
   __shared__ int win_partial;
 
-  const int tid = threadIdx.x;
+  const int tid = tile_tid<kLoop>();
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const TileInfo T = tile_setup(tiles, perm, bands, sm);
+  const TileInfo T = tile_setup(tiles, perm, bands, sm, slot, tid);
   // atom mask bit of this lane's row in row block h / column in column block q
   uint32_t rbit[4] = {0u, 0u, 0u, 0u}, cbit[4] = {0u, 0u, 0u, 0u};
   for (int a = 0; a < T.natI; ++a)
@@ -427,7 +465,7 @@ __global__ __launch_bounds__(256, 2) void numeric_f64_ksplit_kernel(
 
   for (int Q0 = 0; Q0 < T.ktot; Q0 += kTableCols) {
     const int nsteps = build_step_table<4>(T, sm, entries, Q0, col_ab, col_m, step_g, a_zero, b_zero,
-                                        &win_partial);
+                                        &win_partial, tid);
     int cbase = -64;
     uint64_t live = 0;
     uint32_t chunk_bm = 0;
@@ -528,7 +566,39 @@ __global__ __launch_bounds__(256, 2) void numeric_f64_ksplit_kernel(
     __syncthreads();
   }
   tile_epilogue<256>(Ct, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data,
-                     c_data, kept_out);
+                     c_data, kept_out, tid, slot_counter,
+                     next_slot);
+}
+
+template <bool kWhole, bool kLoop>
+__global__ __launch_bounds__(256, 2) void numeric_f64_ksplit_kernel(
+    int32_t n_tiles, int32_t* __restrict__ slot_counter, const TileDesc* __restrict__ tiles,
+    const int32_t* __restrict__ perm,
+    const Entry* __restrict__ entries, const Band* __restrict__ bands,
+    const double* __restrict__ a_pack, const double* __restrict__ b_pack, uint32_t a_zero,
+    uint32_t b_zero, double alpha, double beta, double eps2,
+    const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
+    const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
+    double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
+  LAMB_GUARD;
+  if constexpr (kLoop) {
+    // slots: blockIdx.x first, then the counter's (set to the grid size
+    // before the launch), claimed during each tile's epilogue
+    __shared__ int32_t next_slot;
+    for (int32_t slot = blockIdx.x; slot < n_tiles;) {
+      numeric_f64_ksplit_kernel_tile<kWhole, kLoop>(
+          slot, tiles, perm, entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta,
+          eps2, old_row_p, old_col, old_off, old_data, c_data, kept_out, slot_counter,
+          &next_slot);
+      __syncthreads();  // next_slot is visible; the next set-up rewrites the LDS
+      slot = __builtin_amdgcn_readfirstlane(next_slot);
+    }
+  } else {
+    numeric_f64_ksplit_kernel_tile<kWhole, kLoop>(
+        blockIdx.x, tiles, perm, entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta,
+        eps2, old_row_p, old_col, old_off, old_data, c_data, kept_out, nullptr,
+        nullptr);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -545,16 +615,16 @@ __global__ __launch_bounds__(256, 2) void numeric_f64_ksplit_kernel(
 #endif
 constexpr int kHalfPrefetch = LAMB_HALF_PREFETCH;
 
-template <bool kWhole>
-__global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
-    const TileDesc* __restrict__ tiles, const int32_t* __restrict__ perm,
+template <bool kWhole, bool kLoop>
+__device__ __forceinline__ void numeric_f64_half_kernel_tile(
+    int32_t slot, const TileDesc* __restrict__ tiles, const int32_t* __restrict__ perm,
     const Entry* __restrict__ entries, const Band* __restrict__ bands,
     const double* __restrict__ a_pack, const double* __restrict__ b_pack, uint32_t a_zero,
     uint32_t b_zero, double alpha, double beta, double eps2,
     const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
     const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
-    double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
-  LAMB_GUARD;
+    double* __restrict__ c_data, uint16_t* __restrict__ kept_out,
+    int32_t* __restrict__ slot_counter, int32_t* next_slot) {
   __shared__ double lds[kBandRows * kPitch];
   uint2* col_ab = reinterpret_cast<uint2*>(lds);
   uint32_t* col_m = reinterpret_cast<uint32_t*>(col_ab + kTableCols);
@@ -562,12 +632,12 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
   __shared__ TileSmem sm;
   __shared__ int win_partial;
 
-  const int tid = threadIdx.x;
+  const int tid = tile_tid<kLoop>();
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wc = wave & 1;
   const uint32_t par = (uint32_t)(wave >> 1);
-  const TileInfo T = tile_setup(tiles, perm, bands, sm);
+  const TileInfo T = tile_setup(tiles, perm, bands, sm, slot, tid);
   // atom mask bit of this lane's row in row block h / column in column block q
   uint32_t rbit[4] = {0u, 0u, 0u, 0u}, cbit[2] = {0u, 0u};
   for (int a = 0; a < T.natI; ++a)
@@ -595,7 +665,7 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
 
   for (int Q0 = 0; Q0 < T.ktot; Q0 += kTableCols) {
     const int nsteps = build_step_table<4>(T, sm, entries, Q0, col_ab, col_m, step_g, a_zero,
-                                           b_zero, &win_partial);
+                                           b_zero, &win_partial, tid);
     int cbase = -64;
     uint64_t live = 0;
     uint32_t chunk_bm = 0;
@@ -655,10 +725,13 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
             acc[2 * h + q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[S][h], rb[S][q], acc[2 * h + q], 0, 0, 0); \
     }
 #define HF_SLOTS(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
+    // the masked walk holds per-lane atom masks besides the ring: two steps
+    // in flight keep it within the registers of four waves per SIMD
+    constexpr int kDepth = kWhole ? kHalfPrefetch : (kHalfPrefetch < 2 ? kHalfPrefetch : 2);
     static_assert(kHalfPrefetch >= 1 && kHalfPrefetch <= 8, "prefetch depth 1..8");
-#define HF_PRIME(S) if (S < kHalfPrefetch) HF_FETCH(S)
-#define HF_ANY(S) if (S < kHalfPrefetch) any |= blkmask[S];
-#define HF_STEP(S) if (S < kHalfPrefetch) { HF_MFMA(S); HF_FETCH(S); }
+#define HF_PRIME(S) if (S < kDepth) HF_FETCH(S)
+#define HF_ANY(S) if (S < kDepth) any |= blkmask[S];
+#define HF_STEP(S) if (S < kDepth) { HF_MFMA(S); HF_FETCH(S); }
     HF_SLOTS(HF_PRIME)
     for (;;) {
       uint32_t any = 0;
@@ -695,7 +768,73 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
     __syncthreads();
   }
   tile_epilogue<256>(Ct, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data,
-                     c_data, kept_out);
+                     c_data, kept_out, tid, slot_counter,
+                     next_slot);
+}
+
+template <bool kWhole, bool kLoop>
+__global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
+    int32_t n_tiles, int32_t* __restrict__ slot_counter, const TileDesc* __restrict__ tiles,
+    const int32_t* __restrict__ perm,
+    const Entry* __restrict__ entries, const Band* __restrict__ bands,
+    const double* __restrict__ a_pack, const double* __restrict__ b_pack, uint32_t a_zero,
+    uint32_t b_zero, double alpha, double beta, double eps2,
+    const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
+    const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
+    double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
+  LAMB_GUARD;
+  if constexpr (kLoop) {
+    // slots: blockIdx.x first, then the counter's (set to the grid size
+    // before the launch), claimed during each tile's epilogue
+    __shared__ int32_t next_slot;
+    for (int32_t slot = blockIdx.x; slot < n_tiles;) {
+      numeric_f64_half_kernel_tile<kWhole, kLoop>(
+          slot, tiles, perm, entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta,
+          eps2, old_row_p, old_col, old_off, old_data, c_data, kept_out, slot_counter,
+          &next_slot);
+      __syncthreads();  // next_slot is visible; the next set-up rewrites the LDS
+      slot = __builtin_amdgcn_readfirstlane(next_slot);
+    }
+  } else {
+    numeric_f64_half_kernel_tile<kWhole, kLoop>(
+        blockIdx.x, tiles, perm, entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta,
+        eps2, old_row_p, old_col, old_off, old_data, c_data, kept_out, nullptr,
+        nullptr);
+  }
+}
+
+// Workgroups of a 256-thread kernel resident on the current device at once
+// (resident per CU times the CUs): the grid of a persistent launch. Cached
+// per kernel and device; 0 when the runtime cannot tell.
+int32_t resident_workgroups(const void* kernel) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, int32_t> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  const auto key = std::make_pair(kernel, dev);
+  if (auto it = cache.find(key); it != cache.end()) return it->second;
+  int per_cu = 0, n_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess ||
+      hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  const int32_t n = (int32_t)std::max(0, per_cu) * std::max(0, n_cu);
+  cache[key] = n;
+  return n;
+}
+
+// Sets a persistent grid's slot counter to its first dynamic slot (the grid
+// size: workgroup b takes slot b first). A kernel, not a memset, so that
+// graph captures replay it reliably.
+__global__ void slot_counter_reset(int32_t* __restrict__ counter, int32_t first) {
+  LAMB_GUARD;
+  *counter = first;
+}
+
+void launch_slot_counter_reset(hipStream_t s, int32_t* counter, int32_t first) {
+  hipLaunchKernelGGL(slot_counter_reset, dim3(1), dim3(1), 0, s, counter, first);
 }
 
 hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* tiles,
@@ -704,28 +843,38 @@ hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* ti
                               uint32_t b_zero, double alpha, double beta, double eps2,
                               const int64_t* old_row_p, const int32_t* old_col,
                               const int64_t* old_off, const double* old_data, double* c_data,
-                              uint16_t* kept, int variant, bool whole) {
+                              uint16_t* kept, int variant, bool whole, bool persistent,
+                              int32_t* slot_counter, int32_t* grid_used) {
+  if (grid_used) *grid_used = 0;
   if (n_tiles == 0) return hipSuccess;
+  auto go = [&](auto kernel) {
+    int32_t grid = n_tiles;
+    if (persistent) {
+      const int32_t r = resident_workgroups(reinterpret_cast<const void*>(kernel));
+      if (r > 0) grid = std::min(grid, r);
+      launch_slot_counter_reset(s, slot_counter, grid);
+    }
+    if (grid_used) *grid_used = grid;
+    hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(256), 0, s, n_tiles, slot_counter,
+                       tiles, perm, entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta,
+                       eps2, old_row_p, old_col, old_off, old_data, c_data, kept);
+  };
+  // a persistent grid runs the looping instantiation; one workgroup per tile
+  // the straight one, whose registers are those of a single tile
   if (variant == kF64Ksplit && whole)
-    hipLaunchKernelGGL(numeric_f64_ksplit_kernel<true>, dim3((unsigned)n_tiles), dim3(256), 0, s,
-                       tiles, perm, entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta,
-                       eps2, old_row_p, old_col, old_off, old_data, c_data, kept);
+    persistent ? go(numeric_f64_ksplit_kernel<true, true>)
+               : go(numeric_f64_ksplit_kernel<true, false>);
   else if (variant == kF64Ksplit)
-    hipLaunchKernelGGL(numeric_f64_ksplit_kernel<false>, dim3((unsigned)n_tiles), dim3(256), 0, s,
-                       tiles, perm, entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta,
-                       eps2, old_row_p, old_col, old_off, old_data, c_data, kept);
+    persistent ? go(numeric_f64_ksplit_kernel<false, true>)
+               : go(numeric_f64_ksplit_kernel<false, false>);
   else if (variant == kF64Half && whole)
-    hipLaunchKernelGGL(numeric_f64_half_kernel<true>, dim3((unsigned)n_tiles), dim3(256), 0, s,
-                       tiles, perm, entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta,
-                       eps2, old_row_p, old_col, old_off, old_data, c_data, kept);
+    persistent ? go(numeric_f64_half_kernel<true, true>)
+               : go(numeric_f64_half_kernel<true, false>);
   else if (variant == kF64Half)
-    hipLaunchKernelGGL(numeric_f64_half_kernel<false>, dim3((unsigned)n_tiles), dim3(256), 0, s,
-                       tiles, perm, entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta,
-                       eps2, old_row_p, old_col, old_off, old_data, c_data, kept);
+    persistent ? go(numeric_f64_half_kernel<false, true>)
+               : go(numeric_f64_half_kernel<false, false>);
   else
-    hipLaunchKernelGGL(numeric_f64_kernel, dim3((unsigned)n_tiles), dim3(256), 0, s, tiles, perm,
-                       entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta, eps2,
-                       old_row_p, old_col, old_off, old_data, c_data, kept);
+    persistent ? go(numeric_f64_kernel<true>) : go(numeric_f64_kernel<false>);
   return hipGetLastError();
 }
 

@@ -280,11 +280,14 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
   if (S.r.n_entries > INT32_MAX || S.r.n_tiles > INT32_MAX)
     return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_OVERFLOW_ERROR,
                      "multiply: symbolic sizes exceed 32-bit indices");
-  // FP64 variant (deterministic) and tile order. The autotuner times the
-  // candidate tile orders on the first product of a workload class (form,
+  // FP64 variant (deterministic), tile order and launch form. The autotuner
+  // times the candidate tile orders, each with one workgroup per tile and
+  // with a persistent grid, on the first product of a workload class (form,
   // variant, largest block, half-octave buckets of flops per tile and of C
   // blocks per tile) and keeps the fastest for the context's lifetime; a
-  // graph capture with no tuned class uses 8 row bands per block.
+  // graph capture with no tuned class uses 8 row bands per block and one
+  // workgroup per tile. Every candidate computes every tile the same way, so
+  // the choice never changes the result's bits.
   const int32_t n_tiles = (int32_t)S.r.n_tiles;
   const double nt = std::max<double>(1.0, (double)n_tiles);
   const bool whole = S.r.n_partial == 0;
@@ -297,17 +300,20 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
   const std::array<int64_t, 4> key = {
       (int64_t)flags | ((int64_t)whole << 32) | ((int64_t)variant << 40), d->max_blk,
       bucket(S.r.flops / nt), bucket((double)S.r.n_struct / nt)};
-  int rb = 8;
-  bool tune_order = false;
-  if (const char* env = getenv("LAMB_HIP_TILE_ROWBLOCK")) {
-    rb = std::max(1, std::min(atoi(env), 32));
+  lamb_hip_ctx::TuneChoice choice;
+  bool tune = false;
+  const char* env_rb = getenv("LAMB_HIP_TILE_ROWBLOCK");
+  const char* env_pe = getenv("LAMB_HIP_PERSISTENT");
+  if (env_rb || env_pe) {
+    if (env_rb) choice.rowblock = std::max(1, std::min(atoi(env_rb), 32));
+    if (env_pe) choice.persistent = atoi(env_pe) != 0;
   } else if (auto it = ctx->tune.find(key); it != ctx->tune.end()) {
-    rb = it->second;
+    choice = it->second;
   } else if (ctx->size_mode != kSizesReplay && n_tiles > 0) {
-    tune_order = true;
-    rb = kRowBlocks[0];
+    tune = true;
+    choice.rowblock = kRowBlocks[0];
   }
-  HIPCHK(ctx, build_tile_order(ctx, S.W, D, S.r, rb), "tile order");
+  HIPCHK(ctx, build_tile_order(ctx, S.W, D, S.r, choice.rowblock), "tile order");
   HIPCHK(ctx, record_event(ctx, 3), "event");
 
   // ---- numeric ----
@@ -319,7 +325,10 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
   C->s_data = DevBuf();
   HIPCHK(ctx, newdata.ensure(sizeof(double) * (size_t)std::max<int64_t>(S.r.c_size, 1)),
          "C pool");
-  auto launch_numeric = [&]() -> hipError_t {
+  HIPCHK(ctx, ctx->d_slot_counter.ensure(sizeof(int32_t)), "slot counter");
+  int32_t grid = 0;
+  int32_t* slot_counter = (int32_t*)ctx->d_slot_counter.p;
+  auto launch_numeric = [&](bool persistent) -> hipError_t {
     if (!split)
       return launch_numeric_f64(s, n_tiles, (const TileDesc*)S.r.tiles.p,
                                 (const int32_t*)S.r.perm.p, (const Entry*)S.r.entries.p, D.bands,
@@ -328,41 +337,54 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
                                 use_old ? MC.row_p : nullptr, use_old ? MC.col : nullptr,
                                 use_old ? MC.off : nullptr,
                                 use_old ? (const double*)C->d_data.p : nullptr,
-                                (double*)newdata.p, (uint16_t*)ctx->d_kept.p, variant, whole);
+                                (double*)newdata.p, (uint16_t*)ctx->d_kept.p, variant, whole,
+                                persistent, slot_counter, &grid);
     return launch_numeric_bf16(s, slices, n_tiles, (const TileDesc*)S.r.tiles.p,
                                (const int32_t*)S.r.perm.p, (const Entry*)S.r.entries.p, D.bands,
                                ctx->d_aslice.p, ctx->d_bslice.p, a_zero, b_zero, alpha, beta,
                                post2, use_old ? MC.row_p : nullptr, use_old ? MC.col : nullptr,
                                use_old ? MC.off : nullptr,
                                use_old ? (const double*)C->d_data.p : nullptr,
-                               (double*)newdata.p, (uint16_t*)ctx->d_kept.p);
+                               (double*)newdata.p, (uint16_t*)ctx->d_kept.p, persistent,
+                               slot_counter, &grid);
   };
-  if (tune_order) {
+  if (tune) {
     // every candidate writes the same C pool and kept masks; the last
     // launch's result is the call's, its time the call's numeric time
     float best = 0.f;
-    int best_rb = rb;
+    lamb_hip_ctx::TuneChoice best_choice = choice;
+    int32_t best_grid = 0;
+    bool first = true;
     for (size_t c = 0; c < sizeof(kRowBlocks) / sizeof(kRowBlocks[0]); ++c) {
       if (c > 0) HIPCHK(ctx, build_tile_order(ctx, S.W, D, S.r, kRowBlocks[c]), "tile order");
-      HIPCHK(ctx, record_event(ctx, 0), "event");
-      HIPCHK(ctx, launch_numeric(), "numeric kernel (tuning)");
-      HIPCHK(ctx, record_event(ctx, 1), "event");
-      HIPCHK(ctx, hipEventSynchronize(ctx->ev[1]), "tuning sync");
-      float ms = 0.f;
-      HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]), "event time");
-      if (c == 0 || ms < best) {
-        best = ms;
-        best_rb = kRowBlocks[c];
+      for (int pers = 0; pers < 2; ++pers) {
+        HIPCHK(ctx, record_event(ctx, 0), "event");
+        HIPCHK(ctx, launch_numeric(pers != 0), "numeric kernel (tuning)");
+        HIPCHK(ctx, record_event(ctx, 1), "event");
+        HIPCHK(ctx, hipEventSynchronize(ctx->ev[1]), "tuning sync");
+        float ms = 0.f;
+        HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]), "event time");
+        if (first || ms < best) {
+          best = ms;
+          best_choice.rowblock = kRowBlocks[c];
+          best_choice.persistent = pers != 0;
+          best_grid = grid;
+        }
+        first = false;
       }
     }
-    ctx->tune[key] = best_rb;
-    rb = best_rb;
+    ctx->tune[key] = best_choice;
+    choice = best_choice;
+    grid = best_grid;
   } else {
     HIPCHK(ctx, record_event(ctx, 0), "event");
-    HIPCHK(ctx, launch_numeric(), split ? "numeric kernel (BF16 split)" : "numeric kernel");
+    HIPCHK(ctx, launch_numeric(choice.persistent),
+           split ? "numeric kernel (BF16 split)" : "numeric kernel");
     HIPCHK(ctx, record_event(ctx, 1), "event");
   }
-  ctx->last_rowblock = rb;
+  ctx->last_rowblock = choice.rowblock;
+  ctx->last_grid = grid;
+  ctx->last_tiles = n_tiles;
 
   // ---- C index ----
   CIndex ci;
@@ -575,5 +597,14 @@ extern "C" int32_t lamb_hip_matrix_filter(lamb_hip_mat* C, double eps, int64_t* 
 extern "C" int32_t lamb_hip_tuning_order(const lamb_hip_ctx* ctx, int32_t* last_rowblock) {
   if (!ctx) return LAMB_HIP_FAILURE;
   if (last_rowblock) *last_rowblock = ctx->last_rowblock;
+  return LAMB_HIP_OK;
+}
+
+extern "C" int32_t lamb_hip_tuning_launch(const lamb_hip_ctx* ctx, int32_t* persistent,
+                                          int32_t* workgroups, int32_t* tiles) {
+  if (!ctx) return LAMB_HIP_FAILURE;
+  if (persistent) *persistent = ctx->last_grid >= 0 && ctx->last_grid < ctx->last_tiles;
+  if (workgroups) *workgroups = ctx->last_grid;
+  if (tiles) *tiles = ctx->last_tiles;
   return LAMB_HIP_OK;
 }

@@ -66,6 +66,17 @@ __device__ __forceinline__ double row_totals_dpp(double v, int g) {
   return __builtin_bit_cast(double, r);
 }
 
+// threadIdx.x; with kOpaque, opaque to the optimiser. A persistent kernel's
+// tile loop takes it opaque once per tile, so that per-lane values computed
+// from it stay inside the loop: hoisted out of it, they would be held in
+// registers (or spilled) through the walk.
+template <bool kOpaque>
+__device__ __forceinline__ int tile_tid() {
+  int t = threadIdx.x;
+  if constexpr (kOpaque) asm volatile("" : "+v"(t));
+  return t;
+}
+
 struct TileInfo {
   int tile, I, J, e0, ne, ktot, natI, natJ;
   int32_t firstI, firstJ;
@@ -73,9 +84,11 @@ struct TileInfo {
   const TileDesc* tp;
 };
 
-// Tile of this workgroup and its band geometry in LDS. Workgroup b runs
-// perm[b]. The hardware deals consecutive workgroups round-robin over the 8
-// XCDs, so the tiles running at any moment are a window of the tile order
+// Tile of slot `slot` of the tile order and its band geometry in LDS.
+// Workgroup b runs slots b, b + gridDim.x, ...: one slot with a grid of one
+// workgroup per tile, many with a persistent grid of the resident
+// workgroups. The hardware deals consecutive workgroups round-robin over the
+// 8 XCDs, so the tiles running at any moment are a window of the tile order
 // spread over all XCDs: the A and B strips of that window are read by all of
 // them from the shared last-level cache. (Giving each XCD its own contiguous
 // range of the order, so that each L2 sees a compact range, measured 20.1 ms
@@ -83,10 +96,10 @@ struct TileInfo {
 // threads.
 __device__ __forceinline__ TileInfo tile_setup(const TileDesc* __restrict__ tiles,
                                                const int32_t* __restrict__ perm,
-                                               const Band* __restrict__ bands, TileSmem& sm) {
-  const int tid = threadIdx.x;
+                                               const Band* __restrict__ bands, TileSmem& sm,
+                                               int32_t slot, int tid) {
   TileInfo t;
-  t.tile = perm[blockIdx.x];
+  t.tile = perm[slot];
   t.tp = tiles + t.tile;
   t.I = t.tp->I;
   t.J = t.tp->J;
@@ -169,9 +182,9 @@ __device__ __forceinline__ uint32_t quadrant_blocks(uint32_t g, uint32_t gsh_r, 
   return (r0 & c0) | ((r0 & c1) << 1) | ((r1 & c0) << 2) | ((r1 & c1) << 3);
 }
 
-// Epilogue for a workgroup of NT threads (NT / 64 waves). Ct holds the tile's
-// 64 x 64 accumulated products (row pitch kPitch) and is visible to all
-// threads. Each structural block (i,j) of the tile becomes
+// Epilogue for a workgroup of NT threads (NT / 64 waves; tid = threadIdx.x).
+// Ct holds the tile's 64 x 64 accumulated products (row pitch kPitch) and is
+// visible to all threads. Each structural block (i,j) of the tile becomes
 // alpha * (A*B)_ij + beta * Cold_ij (Cold found by binary search in its CSR
 // row; each operation rounded on its own, no contraction), is written to the
 // C pool at out_base + its offset (blocks back to back in row-major order of
@@ -179,17 +192,24 @@ __device__ __forceinline__ uint32_t quadrant_blocks(uint32_t g, uint32_t gsh_r, 
 // taken by one thread as one sequential FMA chain over the block's values in
 // column-major order, s = fma(x, x, s) from s = +0, the same order as the
 // prefilter's block norms; the block is kept iff that sum is >= eps2.
-// kept_out[tile*16 + i] gets the kept columns of band atom i.
+// kept_out[tile*16 + i] gets the kept columns of band atom i. With a
+// slot_counter (a persistent grid), thread 0 takes the workgroup's next tile
+// slot from it and stores it to *next_slot (LDS), visible after a barrier.
 template <int NT>
 __device__ __forceinline__ void tile_epilogue(
     double* Ct, const TileInfo& t, TileSmem& sm, double alpha, double beta, double eps2,
     const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
     const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
-    double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
+    double* __restrict__ c_data, uint16_t* __restrict__ kept_out, int tid,
+    int32_t* __restrict__ slot_counter = nullptr, int32_t* next_slot = nullptr) {
 #pragma clang fp contract(off)
   static_assert(NT % 64 == 0 && NT >= 64 && NT <= 256, "epilogue threads");
   constexpr int kWaves = NT / 64;
-  const int tid = threadIdx.x, lane = tid & 63;
+  const int lane = tid & 63;
+  // a persistent grid's next tile slot, claimed now and published at the
+  // end, so that the atomic's round trip overlaps the epilogue
+  int32_t claimed = 0;
+  if (slot_counter != nullptr && tid == 0) claimed = atomicAdd(slot_counter, 1);
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   // structural pairs (row-major over band positions: id = 16 ip + jp) and
   // their offsets, NT ids per pass
@@ -343,6 +363,7 @@ __device__ __forceinline__ void tile_epilogue(
   }
   __syncthreads();
   if (tid < kBandAtoms) kept_out[(int64_t)t.tile * kBandAtoms + tid] = (uint16_t)sm.keptw[tid];
+  if (slot_counter != nullptr && tid == 0) *next_slot = claimed;
 }
 
 }  // namespace lamb

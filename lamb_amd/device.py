@@ -253,6 +253,14 @@ def tuning_order(ctx: Context) -> int:
     return rb.value
 
 
+def tuning_launch(ctx: Context):
+    """(persistent?, workgroups launched, C tiles) of the last numeric kernel."""
+    p, w, t = C.c_int32(), C.c_int32(), C.c_int32()
+    ctx.check(ctx.lib.lamb_hip_tuning_launch(ctx.h, C.byref(p), C.byref(w), C.byref(t)),
+              "tuning_launch")
+    return bool(p.value), w.value, t.value
+
+
 def tuning_info(ctx: Context):
     """(FP64 kernel variant of the last multiply, workload classes tuned)."""
     v, n = C.c_int32(), C.c_int32()
