@@ -68,6 +68,9 @@ def parse():
                         "each group's chain under a minute)")
     p.add_argument("--chain-steps", type=int, default=0,
                    help="McWeeny steps (default 20 for config4, 10 for config5)")
+    p.add_argument("--tune-shapes", action="store_true",
+                   help="let the autotuner choose the FP64 tile shape per workload class "
+                        "(lamb_hip_tuning_mode); the shapes it kept are reported")
     p.add_argument("--atoms", type=int, default=3072, help="config5: atoms of the water box")
     p.add_argument("--fd-blocks", type=int, default=0, help="config5: blocks checked by FD")
     p.add_argument("--fd-first", type=int, default=0,
@@ -114,6 +117,22 @@ def cpu_baseline(a, b, eps, every):
         "sample_1_core": f"every {16 * every}th block row of A ({n_rows1} rows), "
                          f"{st1['flops']:.3e} flop in {dt1:.2f} s on one thread",
     }
+
+
+def shape_choices(D, ctx):
+    """{tile shape: classes} of the context's tuning table (shape entries)."""
+    import tempfile
+    names = {0: "quadrant 32x32", 1: "ksplit 64x64", 2: "half 64x32"}
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "tuning.txt")
+        D.tuning_save(ctx, path)
+        lines = open(path).read().splitlines()[1:]
+    out = {}
+    for ln in lines:
+        f = ln.split()
+        if f[0] == "shape":
+            out[names.get(int(f[-1]), f[-1])] = out.get(names.get(int(f[-1]), f[-1]), 0) + 1
+    return out
 
 
 def bf16_leg(D, da, db, dc, dd, eps, args, ms_step64, ms_num64):
@@ -265,6 +284,7 @@ def run_config4_groups(args, bt, steps, eps):
           f"{[len(g) for g in groups]} systems", file=sys.stderr, flush=True)
     n = bt.n_systems
     n_classes = 0
+    shapes = {}
     final_idem = np.full(n, np.nan)
     final_trace = np.full(n, np.nan)
     flops = ms_replay = ms_capture = 0.0
@@ -274,6 +294,7 @@ def run_config4_groups(args, bt, steps, eps):
         gb = S.batch([bt.systems[i] for i in idx])
         dist = batch_distribution(gb.atoms, Distribution)
         ctx = D.Context(0)  # a context per group: its scratch goes with it
+        D.tuning_mode(ctx, args.tune_shapes)
         dd = D.DeviceDistribution(ctx, dist)
         dd.set_groups(gb.group, gb.n_systems)
         dh = D.DeviceMatrix.from_lamb(dd, LambMatrix.from_global(dist, gb.h))
@@ -303,6 +324,8 @@ def run_config4_groups(args, bt, steps, eps):
         for h in (g, out, p0, dh, dd):
             h.close()
         n_classes += D.tuning_info(ctx)[1]
+        for k, v in shape_choices(D, ctx).items():
+            shapes[k] = shapes.get(k, 0) + v
         ctx.close()
     sizes = [a.n_atoms for a, _ in bt.systems]
     return line("batched McWeeny chains: GFLOP/s of the chains replayed as HIP graphs, in "
@@ -318,7 +341,7 @@ def run_config4_groups(args, bt, steps, eps):
                  "final_idempotency_error": final_idem.tolist(),
                  "final_trace": final_trace.tolist(),
                  "max_final_idempotency_error": float(np.nanmax(final_idem)),
-                 "autotuned_classes": n_classes})
+                 "autotuned_classes": n_classes, "tile_shapes_tuned": shapes})
 
 
 def run_config4(args):
@@ -337,6 +360,7 @@ def run_config4(args):
         return run_config4_groups(args, bt, steps, eps)
     dist = batch_distribution(bt.atoms, Distribution)
     ctx = D.Context(0)
+    D.tuning_mode(ctx, args.tune_shapes)
     dd = D.DeviceDistribution(ctx, dist)
     dd.set_groups(bt.group, bt.n_systems)
     dh = D.DeviceMatrix.from_lamb(dd, LambMatrix.from_global(dist, bt.h))
@@ -393,6 +417,7 @@ def run_config4(args):
     for h in (p0, p, dh, dd):
         h.close()
     v, ncls = D.tuning_info(ctx)
+    shapes = shape_choices(D, ctx)
     ctx.close()
     return line("batched McWeeny chains: GFLOP/s of the chain replayed as a HIP graph",
                 flops / (ms_graph * 1e-3) / 1e9, "GFLOP/s", ms_graph, args,
@@ -405,7 +430,7 @@ def run_config4(args):
                  "ms_capture": ms_capture, "graph_equals_eager_bitwise": same,
                  "max_final_idempotency_error": float(idem[-1].max()),
                  "occupied_counts_first_systems": occ[:8].tolist(),
-                 "autotuned_classes": ncls, "bf16_split": bf})
+                 "autotuned_classes": ncls, "tile_shapes_tuned": shapes, "bf16_split": bf})
 
 
 def run_config5(args):
@@ -507,6 +532,7 @@ def main():
     la = LambMatrix.from_global(dist, a)
     lb = LambMatrix.from_global(dist, b)
     ctx = D.Context(0)
+    D.tuning_mode(ctx, args.tune_shapes)
     dd = D.DeviceDistribution(ctx, dist)
     # host sectors -> device pools (the boundary's upload and commit), timed
     # to report the PCIe-inclusive rate beside the device-resident one

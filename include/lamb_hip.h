@@ -229,6 +229,27 @@ int32_t lamb_hip_tuning_info(const lamb_hip_ctx* ctx, int32_t* last_variant,
 /* Row bands per block of the tile order the context's last multiply used (the
  * autotuner's choice for its workload class, or LAMB_HIP_TILE_ROWBLOCK). */
 int32_t lamb_hip_tuning_order(const lamb_hip_ctx* ctx, int32_t* last_rowblock);
+/* Tile-shape tuning (no counterpart in the reference). The per-wave tile of
+ * the FP64 kernel (quadrant 32 x 32, half 64 x 32, K-split 64 x 64; see
+ * lamb_hip_tuning_info) is by default chosen by the rule, because the shapes
+ * add K products in different orders. tune_shapes = 1: the first product of
+ * each workload class (form, whether every entry is whole, the set of block
+ * sizes of the distribution, i.e. the (m, n, k) shapes of its blocks, and
+ * half-octave buckets of flops and of C blocks per tile) times the three
+ * shapes and keeps the fastest for that class; results then depend, at
+ * rounding level, on which shape won. tune_shapes = 0 (default): rule, or a
+ * loaded table's entry. */
+int32_t lamb_hip_tuning_mode(lamb_hip_ctx* ctx, int32_t tune_shapes);
+/* The context's tuning table (tile shapes, tile orders, launch forms per
+ * workload class) written to / read from a text file (no counterpart in the
+ * reference). Loaded entries take effect for their classes as if tuned in
+ * this context: a process that loads the table of an earlier run makes the
+ * same choices, without tuning launches, whatever the mode. A file that is
+ * not a table, or holds a malformed entry, is refused (LAMB_HIP_WRONG_ARGS)
+ * and nothing is loaded; an unreadable or unwritable path gives
+ * LAMB_HIP_EXTERNAL_ERROR. */
+int32_t lamb_hip_tuning_save(const lamb_hip_ctx* ctx, const char* path);
+int32_t lamb_hip_tuning_load(lamb_hip_ctx* ctx, const char* path);
 /* Launch form of the context's last numeric kernel (no counterpart in the
  * reference): persistent = 1 when it ran as a persistent grid, workgroups =
  * the grid launched, tiles = the product's C tiles (-1 before any multiply). */

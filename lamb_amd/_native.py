@@ -98,6 +98,9 @@ SIGNATURES = {
     "lamb_hip_tuning_info": (C.c_int32, [C.c_void_p, i32p, i32p]),
     "lamb_hip_tuning_order": (C.c_int32, [C.c_void_p, i32p]),
     "lamb_hip_tuning_launch": (C.c_int32, [C.c_void_p, i32p, i32p, i32p]),
+    "lamb_hip_tuning_mode": (C.c_int32, [C.c_void_p, C.c_int32]),
+    "lamb_hip_tuning_save": (C.c_int32, [C.c_void_p, C.c_char_p]),
+    "lamb_hip_tuning_load": (C.c_int32, [C.c_void_p, C.c_char_p]),
     "lamb_hip_dist_set_groups": (C.c_int32, [C.c_void_p, C.c_int32, i32p]),
     "lamb_hip_group_trace": (C.c_int32, [C.c_void_p, f64p]),
     "lamb_hip_group_norm": (C.c_int32, [C.c_void_p, f64p]),

@@ -162,6 +162,7 @@ int32_t lamb_hip_dist_create(lamb_hip_ctx* ctx, int64_t n_atoms, const int8_t* b
                        "dist_create: negative block size for atom %d", g + 1);
     d->blk[g] = blk_size[g];
     d->max_blk = std::max(d->max_blk, d->blk[g]);
+    if (d->blk[g] <= 127) d->size_mask[d->blk[g] >> 6] |= 1ull << (d->blk[g] & 63);
     const int32_t s = subset_map[g];
     if (s < 1 || s > n_subsets)
       return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_WRONG_ARGS,

@@ -92,6 +92,10 @@ struct lamb_hip_ctx {
     bool persistent = false;
   };
   std::map<std::array<int64_t, 4>, TuneChoice> tune;
+  // tile shape (FP64 variant: 32 x 32, 64 x 32 or 64 x 64 per wave) per
+  // workload class, when shape tuning is on or a table was loaded
+  int32_t tune_shapes = 0;
+  std::map<std::array<int64_t, 5>, int32_t> shape_tune;
   int32_t last_variant = -1;   // FP64 variant of the last multiply
   int32_t last_rowblock = -1;  // tile-order row bands per block of the last multiply
   int32_t last_grid = -1;      // workgroups of the last numeric launch
@@ -119,6 +123,8 @@ struct lamb_hip_dist {
   int32_t n = 0;
   int32_t n_subsets = 0;
   int32_t max_blk = 0;
+  // block sizes present: bit s of size_mask[s / 64] for size s (0..64)
+  uint64_t size_mask[2] = {0, 0};
   std::vector<int32_t> blk;        // by global atom
   std::vector<int32_t> subset_of;  // 1-based, by global atom
   std::vector<int32_t> local_of;   // 1-based, by global atom

@@ -5,7 +5,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
+#include <cstdio>
 #include <cstring>
+#include <map>
 #include <numeric>
 #include <vector>
 
@@ -280,26 +282,39 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
   if (S.r.n_entries > INT32_MAX || S.r.n_tiles > INT32_MAX)
     return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_OVERFLOW_ERROR,
                      "multiply: symbolic sizes exceed 32-bit indices");
-  // FP64 variant (deterministic), tile order and launch form. The autotuner
-  // times the candidate tile orders, each with one workgroup per tile and
-  // with a persistent grid, on the first product of a workload class (form,
+  // FP64 variant, tile order and launch form. The variant (the per-wave tile
+  // shape) comes from the rule unless shape tuning is on or a loaded table
+  // names it: variants add K products in different orders, so only an
+  // explicit request lets timings choose it. The autotuner then times the
+  // candidate tile orders, each with one workgroup per tile and with a
+  // persistent grid, on the first product of a workload class (form,
   // variant, largest block, half-octave buckets of flops per tile and of C
   // blocks per tile) and keeps the fastest for the context's lifetime; a
   // graph capture with no tuned class uses 8 row bands per block and one
-  // workgroup per tile. Every candidate computes every tile the same way, so
-  // the choice never changes the result's bits.
+  // workgroup per tile. Order and launch form never change the bits.
   const int32_t n_tiles = (int32_t)S.r.n_tiles;
   const double nt = std::max<double>(1.0, (double)n_tiles);
   const bool whole = S.r.n_partial == 0;
   int variant = 0;
+  // the per-wave tile shape (FP64 variant): forced, from the context's shape
+  // table (shape tuning on, or a loaded table), or by the rule. A class is
+  // (form, whole, the distribution's block sizes: the (m, n, k) shapes of
+  // its blocks, half-octave buckets of flops and of C blocks per tile).
+  const std::array<int64_t, 5> shape_key = {
+      (int64_t)flags | ((int64_t)whole << 32), (int64_t)d->size_mask[0],
+      (int64_t)d->size_mask[1], bucket(S.r.flops / nt), bucket((double)S.r.n_struct / nt)};
+  bool tune_shape = false;
   if (!split) {
     const int forced = f64_forced();
-    variant = forced >= 0 ? forced : f64_rule(S.r.flops / nt, whole);
-    ctx->last_variant = variant;
+    if (forced >= 0) {
+      variant = forced;
+    } else if (auto it = ctx->shape_tune.find(shape_key); it != ctx->shape_tune.end()) {
+      variant = it->second;
+    } else {
+      variant = f64_rule(S.r.flops / nt, whole);
+      tune_shape = ctx->tune_shapes != 0 && ctx->size_mode != kSizesReplay && n_tiles > 0;
+    }
   }
-  const std::array<int64_t, 4> key = {
-      (int64_t)flags | ((int64_t)whole << 32) | ((int64_t)variant << 40), d->max_blk,
-      bucket(S.r.flops / nt), bucket((double)S.r.n_struct / nt)};
   lamb_hip_ctx::TuneChoice choice;
   bool tune = false;
   const char* env_rb = getenv("LAMB_HIP_TILE_ROWBLOCK");
@@ -307,11 +322,6 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
   if (env_rb || env_pe) {
     if (env_rb) choice.rowblock = std::max(1, std::min(atoi(env_rb), 32));
     if (env_pe) choice.persistent = atoi(env_pe) != 0;
-  } else if (auto it = ctx->tune.find(key); it != ctx->tune.end()) {
-    choice = it->second;
-  } else if (ctx->size_mode != kSizesReplay && n_tiles > 0) {
-    tune = true;
-    choice.rowblock = kRowBlocks[0];
   }
   HIPCHK(ctx, build_tile_order(ctx, S.W, D, S.r, choice.rowblock), "tile order");
   HIPCHK(ctx, record_event(ctx, 3), "event");
@@ -348,6 +358,46 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
                                (double*)newdata.p, (uint16_t*)ctx->d_kept.p, persistent,
                                slot_counter, &grid);
   };
+  // time one launch of the numeric kernel (every candidate writes the whole
+  // C pool and kept masks; the last launch's result is the call's)
+  auto timed = [&](bool persistent, float* ms) -> hipError_t {
+    hipError_t e;
+    if ((e = record_event(ctx, 0)) != hipSuccess) return e;
+    if ((e = launch_numeric(persistent)) != hipSuccess) return e;
+    if ((e = record_event(ctx, 1)) != hipSuccess) return e;
+    if ((e = hipEventSynchronize(ctx->ev[1])) != hipSuccess) return e;
+    return hipEventElapsedTime(ms, ctx->ev[0], ctx->ev[1]);
+  };
+  if (tune_shape) {
+    // shape tuning: each variant once with the default tile order and one
+    // workgroup per tile; the fastest becomes the class's shape
+    float best = 0.f;
+    int best_v = variant;
+    for (int v : {kF64Half, kF64Quadrant, kF64Ksplit}) {
+      variant = v;
+      float ms = 0.f;
+      HIPCHK(ctx, timed(choice.persistent, &ms), "numeric kernel (shape tuning)");
+      if (v == kF64Half || ms < best) {
+        best = ms;
+        best_v = v;
+      }
+    }
+    variant = best_v;
+    ctx->shape_tune[shape_key] = best_v;
+  }
+  if (!split) ctx->last_variant = variant;
+  const std::array<int64_t, 4> key = {
+      (int64_t)flags | ((int64_t)whole << 32) | ((int64_t)variant << 40), d->max_blk,
+      bucket(S.r.flops / nt), bucket((double)S.r.n_struct / nt)};
+  if (!env_rb && !env_pe) {
+    if (auto it = ctx->tune.find(key); it != ctx->tune.end()) {
+      choice = it->second;
+      if (choice.rowblock != 8)
+        HIPCHK(ctx, build_tile_order(ctx, S.W, D, S.r, choice.rowblock), "tile order");
+    } else if (ctx->size_mode != kSizesReplay && n_tiles > 0) {
+      tune = true;
+    }
+  }
   if (tune) {
     // every candidate writes the same C pool and kept masks; the last
     // launch's result is the call's, its time the call's numeric time
@@ -356,14 +406,10 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
     int32_t best_grid = 0;
     bool first = true;
     for (size_t c = 0; c < sizeof(kRowBlocks) / sizeof(kRowBlocks[0]); ++c) {
-      if (c > 0) HIPCHK(ctx, build_tile_order(ctx, S.W, D, S.r, kRowBlocks[c]), "tile order");
+      HIPCHK(ctx, build_tile_order(ctx, S.W, D, S.r, kRowBlocks[c]), "tile order");
       for (int pers = 0; pers < 2; ++pers) {
-        HIPCHK(ctx, record_event(ctx, 0), "event");
-        HIPCHK(ctx, launch_numeric(pers != 0), "numeric kernel (tuning)");
-        HIPCHK(ctx, record_event(ctx, 1), "event");
-        HIPCHK(ctx, hipEventSynchronize(ctx->ev[1]), "tuning sync");
         float ms = 0.f;
-        HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]), "event time");
+        HIPCHK(ctx, timed(pers != 0, &ms), "numeric kernel (tuning)");
         if (first || ms < best) {
           best = ms;
           best_choice.rowblock = kRowBlocks[c];
@@ -597,6 +643,92 @@ extern "C" int32_t lamb_hip_matrix_filter(lamb_hip_mat* C, double eps, int64_t* 
 extern "C" int32_t lamb_hip_tuning_order(const lamb_hip_ctx* ctx, int32_t* last_rowblock) {
   if (!ctx) return LAMB_HIP_FAILURE;
   if (last_rowblock) *last_rowblock = ctx->last_rowblock;
+  return LAMB_HIP_OK;
+}
+
+extern "C" int32_t lamb_hip_tuning_mode(lamb_hip_ctx* ctx, int32_t tune_shapes) {
+  if (!ctx) return LAMB_HIP_FAILURE;
+  if (tune_shapes != 0 && tune_shapes != 1)
+    return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_WRONG_ARGS,
+                     "tuning_mode: tune_shapes must be 0 or 1, got %d", tune_shapes);
+  ctx->tune_shapes = tune_shapes;
+  return LAMB_HIP_OK;
+}
+
+// Tuning table file: a header line, then one line per tuned class.
+//   shape k0 k1 k2 k3 k4 variant
+//   order k0 k1 k2 k3 rowblock persistent
+static const char kTuneHeader[] = "lamb_hip tuning table 1";
+
+extern "C" int32_t lamb_hip_tuning_save(const lamb_hip_ctx* ctx, const char* path) {
+  if (!ctx) return LAMB_HIP_FAILURE;
+  lamb_hip_ctx* c = const_cast<lamb_hip_ctx*>(ctx);
+  if (!path) return set_error(c, LAMB_HIP_FAILURE, LAMB_HIP_WRONG_ARGS, "tuning_save: NULL path");
+  FILE* f = fopen(path, "w");
+  if (!f)
+    return set_error(c, LAMB_HIP_FAILURE, LAMB_HIP_EXTERNAL_ERROR,
+                     "tuning_save: cannot open %s for writing", path);
+  fprintf(f, "%s\n", kTuneHeader);
+  for (const auto& [k, v] : ctx->shape_tune)
+    fprintf(f, "shape %lld %lld %lld %lld %lld %d\n", (long long)k[0], (long long)k[1],
+            (long long)k[2], (long long)k[3], (long long)k[4], v);
+  for (const auto& [k, v] : ctx->tune)
+    fprintf(f, "order %lld %lld %lld %lld %d %d\n", (long long)k[0], (long long)k[1],
+            (long long)k[2], (long long)k[3], v.rowblock, v.persistent ? 1 : 0);
+  const bool ok = fclose(f) == 0;
+  return ok ? LAMB_HIP_OK
+            : set_error(c, LAMB_HIP_FAILURE, LAMB_HIP_EXTERNAL_ERROR,
+                        "tuning_save: write to %s failed", path);
+}
+
+extern "C" int32_t lamb_hip_tuning_load(lamb_hip_ctx* ctx, const char* path) {
+  if (!ctx) return LAMB_HIP_FAILURE;
+  if (!path) return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_WRONG_ARGS, "tuning_load: NULL path");
+  FILE* f = fopen(path, "r");
+  if (!f)
+    return set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_EXTERNAL_ERROR,
+                     "tuning_load: cannot open %s", path);
+  std::map<std::array<int64_t, 5>, int32_t> shapes;
+  std::map<std::array<int64_t, 4>, lamb_hip_ctx::TuneChoice> orders;
+  char line[512];
+  int lineno = 0;
+  int32_t rc = LAMB_HIP_OK;
+  while (fgets(line, sizeof line, f)) {
+    ++lineno;
+    if (lineno == 1) {
+      if (strncmp(line, kTuneHeader, sizeof kTuneHeader - 1) != 0) {
+        rc = set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_WRONG_ARGS,
+                       "tuning_load: %s is not a tuning table", path);
+        break;
+      }
+      continue;
+    }
+    long long k[5];
+    int a = 0, b = 0;
+    char tail = 0;
+    if (sscanf(line, "shape %lld %lld %lld %lld %lld %d %c", &k[0], &k[1], &k[2], &k[3], &k[4],
+               &a, &tail) == 6 &&
+        (a == kF64Quadrant || a == kF64Ksplit || a == kF64Half)) {
+      shapes[{k[0], k[1], k[2], k[3], k[4]}] = a;
+    } else if (sscanf(line, "order %lld %lld %lld %lld %d %d %c", &k[0], &k[1], &k[2], &k[3], &a,
+                      &b, &tail) == 6 &&
+               a >= 1 && a <= 32 && (b == 0 || b == 1)) {
+      lamb_hip_ctx::TuneChoice t;
+      t.rowblock = a;
+      t.persistent = b != 0;
+      orders[{k[0], k[1], k[2], k[3]}] = t;
+    } else {
+      rc = set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_WRONG_ARGS,
+                     "tuning_load: %s line %d is not a tuning entry", path, lineno);
+      break;
+    }
+  }
+  fclose(f);
+  if (rc == LAMB_HIP_OK && lineno == 0)
+    rc = set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_WRONG_ARGS, "tuning_load: %s is empty", path);
+  if (rc != LAMB_HIP_OK) return rc;
+  for (const auto& [k, v] : shapes) ctx->shape_tune[k] = v;
+  for (const auto& [k, v] : orders) ctx->tune[k] = v;
   return LAMB_HIP_OK;
 }
 

@@ -261,6 +261,21 @@ def tuning_launch(ctx: Context):
     return bool(p.value), w.value, t.value
 
 
+def tuning_mode(ctx: Context, tune_shapes: bool) -> None:
+    """Let the autotuner choose the FP64 tile shape per workload class."""
+    ctx.check(ctx.lib.lamb_hip_tuning_mode(ctx.h, 1 if tune_shapes else 0), "tuning_mode")
+
+
+def tuning_save(ctx: Context, path) -> None:
+    """Write the context's tuning table (shapes, orders, launch forms)."""
+    ctx.check(ctx.lib.lamb_hip_tuning_save(ctx.h, os.fsencode(os.fspath(path))), "tuning_save")
+
+
+def tuning_load(ctx: Context, path) -> None:
+    """Read a tuning table written by tuning_save into the context."""
+    ctx.check(ctx.lib.lamb_hip_tuning_load(ctx.h, os.fsencode(os.fspath(path))), "tuning_load")
+
+
 def tuning_info(ctx: Context):
     """(FP64 kernel variant of the last multiply, workload classes tuned)."""
     v, n = C.c_int32(), C.c_int32()
