@@ -181,182 +181,6 @@ __global__ __launch_bounds__(256) void pack_slices_kernel(
 // number of loads.
 // ---------------------------------------------------------------------------
 template <int S, bool kLoop>
-__device__ __forceinline__ void numeric_bf16_kernel_tile(
-    int32_t slot, const TileDesc* __restrict__ tiles, const int32_t* __restrict__ perm,
-    const Entry* __restrict__ entries, const Band* __restrict__ bands,
-    const uint4* __restrict__ a_sl, const uint4* __restrict__ b_sl, uint32_t a_zero,
-    uint32_t b_zero, double alpha, double beta, double eps2,
-    const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
-    const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
-    double* __restrict__ c_data, uint16_t* __restrict__ kept_out,
-    int32_t* __restrict__ slot_counter, int32_t* next_slot) {
-  __shared__ double lds[kBandRows * kPitch];
-  uint2* tbl_ab = reinterpret_cast<uint2*>(lds);  // A, B chunk
-  uint32_t* tbl_m = reinterpret_cast<uint32_t*>(tbl_ab + kTableChunks);  // rmask | cmask << 16
-  uint32_t* step_g = tbl_m + kTableChunks;  // row groups | col groups << 16
-  static_assert(sizeof(uint32_t) * (3 * kTableChunks + kTableChunks / 4) <=
-                    sizeof(double) * kBandRows * kPitch, "chunk table exceeds LDS buffer");
-  __shared__ TileSmem sm;
-  __shared__ int32_t scan_w[4];
-  // FP64 sums of the finished chains, one element per lane of one wave (the
-  // wave quadrants are disjoint), read by the epilogue
-  __shared__ double ctile[kBandRows * kPitch];
-
-  const int tid = tile_tid<kLoop>();
-  const int lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int wr = wave >> 1, wc = wave & 1;
-  const TileInfo T = tile_setup(tiles, perm, bands, sm, slot, tid);
-  uint32_t rbit[2], cbit[2];
-  lane_atom_bits(T, sm, wr, wc, lane, rbit, cbit);
-  const uint32_t gsh_r = 8 * wr, gsh_c = 8 * wc;
-  const int64_t a_stride = (int64_t)a_zero + 1, b_stride = (int64_t)b_zero + 1;
-  const uint4* a_lane = a_sl + 32 * wr + (lane & 15);
-  const uint4* b_lane = b_sl + 32 * wc + (lane & 15);
-
-  // FP32 level accumulators [level 2, 3, 4, 5+][block 2h+q] (levels past
-  // S+1 unused). v_mfma_f32_16x16x32_bf16 result: D[i][j] in lane
-  // 16*(i/4) + j, register i%4; flush() adds them into ctile and clears them.
-  constexpr int NA = S < 4 ? S : 4;
-  f32x4 acc[NA][4];
-#pragma unroll
-  for (int b = 0; b < 4; ++b)
-#pragma unroll
-    for (int L = 0; L < NA; ++L) acc[L][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int t = tid; t < kBandRows * kPitch; t += 256) ctile[t] = 0.0;
-  __syncthreads();  // zeros visible before any wave flushes into its quadrant
-  auto flush = [&]() {
-#pragma unroll
-    for (int h = 0; h < 2; ++h)
-#pragma unroll
-      for (int q = 0; q < 2; ++q)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = 32 * wr + 16 * h + 4 * (lane >> 4) + r;
-          const int col = 32 * wc + 16 * q + (lane & 15);
-          double v = ctile[row * kPitch + col];
-#pragma unroll
-          for (int L = 0; L < NA; ++L) v += (double)acc[L][2 * h + q][r];
-          ctile[row * kPitch + col] = v;
-        }
-#pragma unroll
-    for (int b = 0; b < 4; ++b)
-#pragma unroll
-      for (int L = 0; L < NA; ++L) acc[L][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-  };
-  int chain = 0;  // live steps in the current chain (wave-uniform)
-
-  for (int E0 = 0; E0 < T.ne; E0 += 256) {
-    // ---- chunk offsets of this window's entries (block scan) ----
-    const int e = E0 + tid;
-    int nch = 0;
-    Entry en;
-    int ks = 0;
-    if (e < T.ne) {
-      en = entries[T.e0 + e];
-      ks = en.kstart;
-      const int ke = (e + 1 < T.ne) ? entries[T.e0 + e + 1].kstart : T.ktot;
-      nch = (ke - ks + kChunk - 1) / kChunk;
-    }
-    int incl = nch;
-    for (int o = 1; o < 64; o <<= 1) {
-      const int y = __shfl_up(incl, o, 64);
-      if (lane >= o) incl += y;
-    }
-    __syncthreads();  // previous window's walk is done with the table
-    if (lane == 63) scan_w[wave] = incl;
-    __syncthreads();
-    int base = 0;
-    for (int w = 0; w < wave; ++w) base += scan_w[w];
-    const int nchunks = scan_w[0] + scan_w[1] + scan_w[2] + scan_w[3];
-    const int nsteps = (nchunks + 3) >> 2;
-    for (int t = nchunks + tid; t < 4 * nsteps; t += 256) {
-      tbl_ab[t] = make_uint2(a_zero, b_zero);
-      tbl_m[t] = 0u;
-    }
-    for (int t = tid; t < nsteps; t += 256) step_g[t] = 0;
-    __syncthreads();
-    if (nch) {
-      const int c0 = base + incl - nch;
-      const uint32_t m = (uint32_t)en.rmask | ((uint32_t)en.cmask << 16);
-      const uint32_t g = entry_groups(sm, en.rmask, en.cmask);
-      const uint32_t ac = (en.a_col & ~kEntryPartial) / kChunk, bc = en.b_col / kChunk;
-      for (int c = 0; c < nch; ++c) {
-        tbl_ab[c0 + c] = make_uint2(ac + c, bc + c);
-        tbl_m[c0 + c] = m;
-      }
-      for (int st = c0 >> 2; st <= (c0 + nch - 1) >> 2; ++st) atomicOr(&step_g[st], g);
-    }
-    __syncthreads();
-
-    // ---- per-wave walk over the window's live steps ----
-    int cbase = -64;
-    uint64_t live = 0;
-    uint32_t chunk_bm = 0;
-    for (;;) {
-      while (live == 0 && cbase + 64 < nsteps) {
-        cbase += 64;
-        const int sl = cbase + lane;
-        chunk_bm = quadrant_blocks(sl < nsteps ? step_g[sl] : 0u, gsh_r, gsh_c);
-        live = __ballot(chunk_bm != 0u);
-      }
-      if (live == 0) break;
-      const int l = __builtin_ctzll(live);
-      live &= live - 1;
-      const int st = cbase + l;
-      const uint32_t bm = __builtin_amdgcn_readlane(chunk_bm, l);
-      const int Qc = 4 * st + (lane >> 4);
-      const uint2 ab = tbl_ab[Qc];
-      const uint32_t cm = tbl_m[Qc];
-      bf16x8 ra[S][2], rb[S][2];
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const uint32_t need = (bm & (3u << (2 * h))) ? rbit[h] : 0u;
-        const int64_t c = (cm & need) ? ab.x : a_zero;
-#pragma unroll
-        for (int t = 0; t < S; ++t) {
-          const uint4 v = a_lane[(t * a_stride + c) * 64 + 16 * h];
-          ra[t][h] = __builtin_bit_cast(bf16x8, v);
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        const uint32_t need = (bm & (5u << q)) ? cbit[q] : 0u;
-        const int64_t c = (cm & need) ? ab.y : b_zero;
-#pragma unroll
-        for (int u = 0; u < S; ++u) {
-          const uint4 v = b_lane[(u * b_stride + c) * 64 + 16 * q];
-          rb[u][q] = __builtin_bit_cast(bf16x8, v);
-        }
-      }
-#pragma unroll
-      for (int h = 0; h < 2; ++h)
-#pragma unroll
-        for (int q = 0; q < 2; ++q)
-          if ((bm >> (2 * h + q)) & 1u) {
-#pragma unroll
-            for (int t = 0; t < S; ++t)
-#pragma unroll
-              for (int u = 0; u < S - t; ++u) {
-                const int L = t + u < NA - 1 ? t + u : NA - 1;
-                acc[L][2 * h + q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                    ra[t][h], rb[u][q], acc[L][2 * h + q], 0, 0, 0);
-              }
-          }
-      if (++chain == kChainSteps) {
-        chain = 0;
-        flush();
-      }
-    }
-  }
-  flush();
-  __syncthreads();
-  tile_epilogue<256>(ctile, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data,
-                     c_data, kept_out, tid, slot_counter,
-                     next_slot);
-}
-
-template <int S, bool kLoop>
 __global__ __launch_bounds__(256, 2) void numeric_bf16_kernel(
     int32_t n_tiles, int32_t* __restrict__ slot_counter, const TileDesc* __restrict__ tiles,
     const int32_t* __restrict__ perm,
@@ -367,23 +191,179 @@ __global__ __launch_bounds__(256, 2) void numeric_bf16_kernel(
     const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
     double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
   LAMB_GUARD;
-  if constexpr (kLoop) {
-    // slots: blockIdx.x first, then the counter's (set to the grid size
-    // before the launch), claimed during each tile's epilogue
-    __shared__ int32_t next_slot;
-    for (int32_t slot = blockIdx.x; slot < n_tiles;) {
-      numeric_bf16_kernel_tile<S, kLoop>(
-          slot, tiles, perm, entries, bands, a_sl, b_sl, a_zero, b_zero, alpha, beta, eps2,
-          old_row_p, old_col, old_off, old_data, c_data, kept_out, slot_counter,
-          &next_slot);
-      __syncthreads();  // next_slot is visible; the next set-up rewrites the LDS
-      slot = __builtin_amdgcn_readfirstlane(next_slot);
+  // slots: blockIdx.x first; with kLoop (a persistent grid) the next ones
+  // from the counter (set to the grid size before the launch), claimed
+  // during each tile's epilogue. Without kLoop the body runs once, as a
+  // plain one-tile kernel with that kernel's registers.
+  __shared__ int32_t next_slot;
+  for (int32_t slot = blockIdx.x;;) {
+    __shared__ double lds[kBandRows * kPitch];
+    uint2* tbl_ab = reinterpret_cast<uint2*>(lds);  // A, B chunk
+    uint32_t* tbl_m = reinterpret_cast<uint32_t*>(tbl_ab + kTableChunks);  // rmask | cmask << 16
+    uint32_t* step_g = tbl_m + kTableChunks;  // row groups | col groups << 16
+    static_assert(sizeof(uint32_t) * (3 * kTableChunks + kTableChunks / 4) <=
+                      sizeof(double) * kBandRows * kPitch, "chunk table exceeds LDS buffer");
+    __shared__ TileSmem sm;
+    __shared__ int32_t scan_w[4];
+    // FP64 sums of the finished chains, one element per lane of one wave (the
+    // wave quadrants are disjoint), read by the epilogue
+    __shared__ double ctile[kBandRows * kPitch];
+
+    const int tid = tile_tid<kLoop>();
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wr = wave >> 1, wc = wave & 1;
+    const TileInfo T = tile_setup(tiles, perm, bands, sm, slot, tid);
+    uint32_t rbit[2], cbit[2];
+    lane_atom_bits(T, sm, wr, wc, lane, rbit, cbit);
+    const uint32_t gsh_r = 8 * wr, gsh_c = 8 * wc;
+    const int64_t a_stride = (int64_t)a_zero + 1, b_stride = (int64_t)b_zero + 1;
+    const uint4* a_lane = a_sl + 32 * wr + (lane & 15);
+    const uint4* b_lane = b_sl + 32 * wc + (lane & 15);
+
+    // FP32 level accumulators [level 2, 3, 4, 5+][block 2h+q] (levels past
+    // S+1 unused). v_mfma_f32_16x16x32_bf16 result: D[i][j] in lane
+    // 16*(i/4) + j, register i%4; flush() adds them into ctile and clears them.
+    constexpr int NA = S < 4 ? S : 4;
+    f32x4 acc[NA][4];
+  #pragma unroll
+    for (int b = 0; b < 4; ++b)
+  #pragma unroll
+      for (int L = 0; L < NA; ++L) acc[L][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int t = tid; t < kBandRows * kPitch; t += 256) ctile[t] = 0.0;
+    __syncthreads();  // zeros visible before any wave flushes into its quadrant
+    auto flush = [&]() {
+  #pragma unroll
+      for (int h = 0; h < 2; ++h)
+  #pragma unroll
+        for (int q = 0; q < 2; ++q)
+  #pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = 32 * wr + 16 * h + 4 * (lane >> 4) + r;
+            const int col = 32 * wc + 16 * q + (lane & 15);
+            double v = ctile[row * kPitch + col];
+  #pragma unroll
+            for (int L = 0; L < NA; ++L) v += (double)acc[L][2 * h + q][r];
+            ctile[row * kPitch + col] = v;
+          }
+  #pragma unroll
+      for (int b = 0; b < 4; ++b)
+  #pragma unroll
+        for (int L = 0; L < NA; ++L) acc[L][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    };
+    int chain = 0;  // live steps in the current chain (wave-uniform)
+
+    for (int E0 = 0; E0 < T.ne; E0 += 256) {
+      // ---- chunk offsets of this window's entries (block scan) ----
+      const int e = E0 + tid;
+      int nch = 0;
+      Entry en;
+      int ks = 0;
+      if (e < T.ne) {
+        en = entries[T.e0 + e];
+        ks = en.kstart;
+        const int ke = (e + 1 < T.ne) ? entries[T.e0 + e + 1].kstart : T.ktot;
+        nch = (ke - ks + kChunk - 1) / kChunk;
+      }
+      int incl = nch;
+      for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+      }
+      __syncthreads();  // previous window's walk is done with the table
+      if (lane == 63) scan_w[wave] = incl;
+      __syncthreads();
+      int base = 0;
+      for (int w = 0; w < wave; ++w) base += scan_w[w];
+      const int nchunks = scan_w[0] + scan_w[1] + scan_w[2] + scan_w[3];
+      const int nsteps = (nchunks + 3) >> 2;
+      for (int t = nchunks + tid; t < 4 * nsteps; t += 256) {
+        tbl_ab[t] = make_uint2(a_zero, b_zero);
+        tbl_m[t] = 0u;
+      }
+      for (int t = tid; t < nsteps; t += 256) step_g[t] = 0;
+      __syncthreads();
+      if (nch) {
+        const int c0 = base + incl - nch;
+        const uint32_t m = (uint32_t)en.rmask | ((uint32_t)en.cmask << 16);
+        const uint32_t g = entry_groups(sm, en.rmask, en.cmask);
+        const uint32_t ac = (en.a_col & ~kEntryPartial) / kChunk, bc = en.b_col / kChunk;
+        for (int c = 0; c < nch; ++c) {
+          tbl_ab[c0 + c] = make_uint2(ac + c, bc + c);
+          tbl_m[c0 + c] = m;
+        }
+        for (int st = c0 >> 2; st <= (c0 + nch - 1) >> 2; ++st) atomicOr(&step_g[st], g);
+      }
+      __syncthreads();
+
+      // ---- per-wave walk over the window's live steps ----
+      int cbase = -64;
+      uint64_t live = 0;
+      uint32_t chunk_bm = 0;
+      for (;;) {
+        while (live == 0 && cbase + 64 < nsteps) {
+          cbase += 64;
+          const int sl = cbase + lane;
+          chunk_bm = quadrant_blocks(sl < nsteps ? step_g[sl] : 0u, gsh_r, gsh_c);
+          live = __ballot(chunk_bm != 0u);
+        }
+        if (live == 0) break;
+        const int l = __builtin_ctzll(live);
+        live &= live - 1;
+        const int st = cbase + l;
+        const uint32_t bm = __builtin_amdgcn_readlane(chunk_bm, l);
+        const int Qc = 4 * st + (lane >> 4);
+        const uint2 ab = tbl_ab[Qc];
+        const uint32_t cm = tbl_m[Qc];
+        bf16x8 ra[S][2], rb[S][2];
+  #pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const uint32_t need = (bm & (3u << (2 * h))) ? rbit[h] : 0u;
+          const int64_t c = (cm & need) ? ab.x : a_zero;
+  #pragma unroll
+          for (int t = 0; t < S; ++t) {
+            const uint4 v = a_lane[(t * a_stride + c) * 64 + 16 * h];
+            ra[t][h] = __builtin_bit_cast(bf16x8, v);
+          }
+        }
+  #pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const uint32_t need = (bm & (5u << q)) ? cbit[q] : 0u;
+          const int64_t c = (cm & need) ? ab.y : b_zero;
+  #pragma unroll
+          for (int u = 0; u < S; ++u) {
+            const uint4 v = b_lane[(u * b_stride + c) * 64 + 16 * q];
+            rb[u][q] = __builtin_bit_cast(bf16x8, v);
+          }
+        }
+  #pragma unroll
+        for (int h = 0; h < 2; ++h)
+  #pragma unroll
+          for (int q = 0; q < 2; ++q)
+            if ((bm >> (2 * h + q)) & 1u) {
+  #pragma unroll
+              for (int t = 0; t < S; ++t)
+  #pragma unroll
+                for (int u = 0; u < S - t; ++u) {
+                  const int L = t + u < NA - 1 ? t + u : NA - 1;
+                  acc[L][2 * h + q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                      ra[t][h], rb[u][q], acc[L][2 * h + q], 0, 0, 0);
+                }
+            }
+        if (++chain == kChainSteps) {
+          chain = 0;
+          flush();
+        }
+      }
     }
-  } else {
-    numeric_bf16_kernel_tile<S, kLoop>(
-        blockIdx.x, tiles, perm, entries, bands, a_sl, b_sl, a_zero, b_zero, alpha, beta, eps2,
-        old_row_p, old_col, old_off, old_data, c_data, kept_out, nullptr,
-        nullptr);
+    flush();
+    __syncthreads();
+    tile_epilogue<256>(ctile, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data,
+                       c_data, kept_out, tid, kLoop ? slot_counter : nullptr, &next_slot);
+    if constexpr (!kLoop) break;
+    __syncthreads();  // next_slot is visible; the next set-up rewrites the LDS
+    slot = __builtin_amdgcn_readfirstlane(next_slot);
+    if (slot >= n_tiles) break;
   }
 }
 
@@ -447,9 +427,12 @@ hipError_t launch_numeric_bf16(hipStream_t s, int slices, int32_t n_tiles, const
   };
   // a persistent grid runs the looping instantiation
   switch (slices) {
-    case 3: persistent ? go(numeric_bf16_kernel<3, true>) : go(numeric_bf16_kernel<3, false>); break;
-    case 4: persistent ? go(numeric_bf16_kernel<4, true>) : go(numeric_bf16_kernel<4, false>); break;
-    case 5: persistent ? go(numeric_bf16_kernel<5, true>) : go(numeric_bf16_kernel<5, false>); break;
+    case 3: persistent ? go(numeric_bf16_kernel<3, true>) : go(numeric_bf16_kernel<3, false>);
+      break;
+    case 4: persistent ? go(numeric_bf16_kernel<4, true>) : go(numeric_bf16_kernel<4, false>);
+      break;
+    case 5: persistent ? go(numeric_bf16_kernel<5, true>) : go(numeric_bf16_kernel<5, false>);
+      break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

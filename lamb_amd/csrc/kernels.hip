@@ -200,163 +200,6 @@ __device__ __forceinline__ int build_step_table(const TileInfo& T, const TileSme
 }
 
 template <bool kLoop>
-__device__ __forceinline__ void numeric_f64_kernel_tile(
-    int32_t slot, const TileDesc* __restrict__ tiles, const int32_t* __restrict__ perm,
-    const Entry* __restrict__ entries, const Band* __restrict__ bands,
-    const double* __restrict__ a_pack, const double* __restrict__ b_pack, uint32_t a_zero,
-    uint32_t b_zero, double alpha, double beta, double eps2,
-    const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
-    const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
-    double* __restrict__ c_data, uint16_t* __restrict__ kept_out,
-    int32_t* __restrict__ slot_counter, int32_t* next_slot) {
-  // main loop: the step table of a window of K columns; epilogue: C tile
-  __shared__ double lds[kBandRows * kPitch];
-  uint2* col_ab = reinterpret_cast<uint2*>(lds);  // A, B panel column
-  uint32_t* col_m = reinterpret_cast<uint32_t*>(col_ab + kTableCols);  // rmask | cmask << 16
-  uint32_t* step_g = col_m + kTableCols;  // row groups | col groups << 16
-  static_assert(sizeof(uint32_t) * (3 * kTableCols + kTableCols / 4) <=
-                    sizeof(double) * kBandRows * kPitch, "step table exceeds LDS buffer");
-  __shared__ TileSmem sm;
-
-  const int tid = tile_tid<kLoop>();
-  const int lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int wr = wave >> 1, wc = wave & 1;
-  const TileInfo T = tile_setup(tiles, perm, bands, sm, slot, tid);
-  const int e0 = T.e0, ne = T.ne, ktot = T.ktot;
-  uint32_t rbit[2], cbit[2];
-  lane_atom_bits(T, sm, wr, wc, lane, rbit, cbit);
-  const int kc = lane >> 4;  // K column of this lane within a step
-  // the 16-bit row / column group masks (4 rows or columns per group) of
-  // this wave's halves and quads: rows 32wr+16h.., cols 32wc+16q..
-  const uint32_t gsh_r = 8 * wr, gsh_c = 8 * wc;
-  const double* a_lane = a_pack + 32 * wr + (lane & 15);
-  const double* b_lane = b_pack + 32 * wc + (lane & 15);
-
-  dbl4 acc[4];
-#pragma unroll
-  for (int x = 0; x < 4; ++x) acc[x] = dbl4{0.0, 0.0, 0.0, 0.0};
-
-  __shared__ int win_partial;
-  // the walk over one window's live steps, kPrefetch ahead; masked: lanes
-  // test their atom against each K column's entry masks (windows holding a
-  // partial entry), else whole panel columns are loaded (the panels are zero
-  // outside the strips' atoms, which then equal the entries' atoms)
-  auto walk = [&](int nsteps, auto masked_tag) {
-    constexpr bool kMasked = decltype(masked_tag)::value;
-    int cbase = -64;        // first step of the current chunk
-    uint64_t live = 0;      // live steps of the chunk not yet taken
-    uint32_t chunk_bm = 0;  // lane l: block mask of step cbase + l
-    // slot S holds, per group g of 4 K columns, A halves [2g], [2g+1] and
-    // B quads [2g], [2g+1] (slots 0..kPrefetch-1 used)
-    double ra[8][2 * kGroups], rb[8][2 * kGroups];
-    uint32_t blkmask[8];
-#define LAMB_FETCH(S)                                                              \
-    {                                                                              \
-      while (live == 0 && cbase + 64 < nsteps) {                                   \
-        cbase += 64;                                                               \
-        const int sl = cbase + lane;                                               \
-        chunk_bm = quadrant_blocks(sl < nsteps ? step_g[sl] : 0u, gsh_r, gsh_c);   \
-        live = __ballot(chunk_bm != 0u);                                           \
-      }                                                                            \
-      uint32_t bm = 0;                                                             \
-      int st = 0;                                                                  \
-      if (live) {                                                                  \
-        const int l = __builtin_ctzll(live);                                       \
-        live &= live - 1;                                                          \
-        st = cbase + l;                                                            \
-        bm = __builtin_amdgcn_readlane(chunk_bm, l);                               \
-      }                                                                            \
-      blkmask[S] = bm;                                                             \
-      _Pragma("unroll") for (int g = 0; g < kGroups; ++g) {                        \
-        const int Q = kStepCols * st + 4 * g + kc;                                 \
-        const uint2 ab = col_ab[Q];                                                \
-        if constexpr (kMasked) {                                                   \
-          const uint32_t cm = col_m[Q];                                            \
-          _Pragma("unroll") for (int h = 0; h < 2; ++h) {                          \
-            const uint32_t need = (bm & (3u << (2 * h))) ? rbit[h] : 0u;           \
-            const uint32_t c = (cm & need) ? ab.x : a_zero;                        \
-            ra[S][2 * g + h] = a_lane[(int64_t)c * 64 + 16 * h];                   \
-          }                                                                        \
-          _Pragma("unroll") for (int q = 0; q < 2; ++q) {                          \
-            const uint32_t need = (bm & (5u << q)) ? cbit[q] : 0u;                 \
-            const uint32_t c = (cm & need) ? ab.y : b_zero;                        \
-            rb[S][2 * g + q] = b_lane[(int64_t)c * 64 + 16 * q];                   \
-          }                                                                        \
-        } else {                                                                   \
-          /* past the last live step the loads read step 0 and feed no MFMA */     \
-          const double* pa = a_lane + (int64_t)ab.x * 64;                          \
-          const double* pb = b_lane + (int64_t)ab.y * 64;                          \
-          ra[S][2 * g] = pa[0];                                                    \
-          ra[S][2 * g + 1] = pa[16];                                               \
-          rb[S][2 * g] = pb[0];                                                    \
-          rb[S][2 * g + 1] = pb[16];                                               \
-        }                                                                          \
-      }                                                                            \
-    }
-    // the groups' MFMAs interleave over the four accumulators
-#define LAMB_MFMA(S)                                                               \
-    {                                                                              \
-      const uint32_t bm = blkmask[S];                                              \
-      _Pragma("unroll") for (int g = 0; g < kGroups; ++g)                          \
-        _Pragma("unroll") for (int h = 0; h < 2; ++h)                              \
-          _Pragma("unroll") for (int q = 0; q < 2; ++q)                            \
-            if ((bm >> (2 * h + q)) & 1u)                                          \
-              acc[2 * h + q] = __builtin_amdgcn_mfma_f64_16x16x4f64(               \
-                  ra[S][2 * g + h], rb[S][2 * g + q], acc[2 * h + q], 0, 0, 0);    \
-    }
-    // slots spelled out so that every register array index is a constant
-#define LAMB_SLOTS(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
-    static_assert(kPrefetch >= 1 && kPrefetch <= 8, "prefetch depth 1..8");
-#define LAMB_PRIME(S) if (S < kPrefetch) LAMB_FETCH(S)
-#define LAMB_ANY(S) if (S < kPrefetch) any |= blkmask[S];
-#define LAMB_STEP(S) if (S < kPrefetch) { LAMB_MFMA(S); LAMB_FETCH(S); }
-    LAMB_SLOTS(LAMB_PRIME)
-    for (;;) {
-      uint32_t any = 0;
-      LAMB_SLOTS(LAMB_ANY)
-      if (!any) break;
-      LAMB_SLOTS(LAMB_STEP)
-    }
-#undef LAMB_STEP
-#undef LAMB_ANY
-#undef LAMB_PRIME
-#undef LAMB_SLOTS
-#undef LAMB_FETCH
-#undef LAMB_MFMA
-  };
-
-  for (int Q0 = 0; Q0 < ktot; Q0 += kTableCols) {
-    const int nsteps = build_step_table<kStepCols>(T, sm, entries, Q0, col_ab, col_m, step_g,
-                                                   a_zero, b_zero,
-                                        &win_partial, tid);
-    if (win_partial)
-      walk(nsteps, std::true_type{});
-    else
-      walk(nsteps, std::false_type{});
-  }
-  __syncthreads();
-
-  // ---- epilogue: C tile to LDS ----
-  // v_mfma_f64_16x16x4 result: D[i][j] in lane 16*(i%4) + j, register i/4.
-  double* Ct = lds;
-#pragma unroll
-  for (int h = 0; h < 2; ++h)
-#pragma unroll
-    for (int q = 0; q < 2; ++q)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = 32 * wr + 16 * h + 4 * r + (lane >> 4);
-        const int col = 32 * wc + 16 * q + (lane & 15);
-        Ct[row * kPitch + col] = acc[2 * h + q][r];
-      }
-  __syncthreads();
-  tile_epilogue<256>(Ct, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data,
-                     c_data, kept_out, tid, slot_counter,
-                     next_slot);
-}
-
-template <bool kLoop>
 __global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
     int32_t n_tiles, int32_t* __restrict__ slot_counter, const TileDesc* __restrict__ tiles,
     const int32_t* __restrict__ perm,
@@ -367,23 +210,160 @@ __global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
     const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
     double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
   LAMB_GUARD;
-  if constexpr (kLoop) {
-    // slots: blockIdx.x first, then the counter's (set to the grid size
-    // before the launch), claimed during each tile's epilogue
-    __shared__ int32_t next_slot;
-    for (int32_t slot = blockIdx.x; slot < n_tiles;) {
-      numeric_f64_kernel_tile<kLoop>(
-          slot, tiles, perm, entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta,
-          eps2, old_row_p, old_col, old_off, old_data, c_data, kept_out, slot_counter,
-          &next_slot);
-      __syncthreads();  // next_slot is visible; the next set-up rewrites the LDS
-      slot = __builtin_amdgcn_readfirstlane(next_slot);
+  // slots: blockIdx.x first; with kLoop (a persistent grid) the next ones
+  // from the counter (set to the grid size before the launch), claimed
+  // during each tile's epilogue. Without kLoop the body runs once, as a
+  // plain one-tile kernel with that kernel's registers.
+  __shared__ int32_t next_slot;
+  for (int32_t slot = blockIdx.x;;) {
+    // main loop: the step table of a window of K columns; epilogue: C tile
+    __shared__ double lds[kBandRows * kPitch];
+    uint2* col_ab = reinterpret_cast<uint2*>(lds);  // A, B panel column
+    uint32_t* col_m = reinterpret_cast<uint32_t*>(col_ab + kTableCols);  // rmask | cmask << 16
+    uint32_t* step_g = col_m + kTableCols;  // row groups | col groups << 16
+    static_assert(sizeof(uint32_t) * (3 * kTableCols + kTableCols / 4) <=
+                      sizeof(double) * kBandRows * kPitch, "step table exceeds LDS buffer");
+    __shared__ TileSmem sm;
+
+    const int tid = tile_tid<kLoop>();
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wr = wave >> 1, wc = wave & 1;
+    const TileInfo T = tile_setup(tiles, perm, bands, sm, slot, tid);
+    const int e0 = T.e0, ne = T.ne, ktot = T.ktot;
+    uint32_t rbit[2], cbit[2];
+    lane_atom_bits(T, sm, wr, wc, lane, rbit, cbit);
+    const int kc = lane >> 4;  // K column of this lane within a step
+    // the 16-bit row / column group masks (4 rows or columns per group) of
+    // this wave's halves and quads: rows 32wr+16h.., cols 32wc+16q..
+    const uint32_t gsh_r = 8 * wr, gsh_c = 8 * wc;
+    const double* a_lane = a_pack + 32 * wr + (lane & 15);
+    const double* b_lane = b_pack + 32 * wc + (lane & 15);
+
+    dbl4 acc[4];
+  #pragma unroll
+    for (int x = 0; x < 4; ++x) acc[x] = dbl4{0.0, 0.0, 0.0, 0.0};
+
+    __shared__ int win_partial;
+    // the walk over one window's live steps, kPrefetch ahead; masked: lanes
+    // test their atom against each K column's entry masks (windows holding a
+    // partial entry), else whole panel columns are loaded (the panels are zero
+    // outside the strips' atoms, which then equal the entries' atoms)
+    auto walk = [&](int nsteps, auto masked_tag) {
+      constexpr bool kMasked = decltype(masked_tag)::value;
+      int cbase = -64;        // first step of the current chunk
+      uint64_t live = 0;      // live steps of the chunk not yet taken
+      uint32_t chunk_bm = 0;  // lane l: block mask of step cbase + l
+      // slot S holds, per group g of 4 K columns, A halves [2g], [2g+1] and
+      // B quads [2g], [2g+1] (slots 0..kPrefetch-1 used)
+      double ra[8][2 * kGroups], rb[8][2 * kGroups];
+      uint32_t blkmask[8];
+  #define LAMB_FETCH(S)                                                              \
+      {                                                                              \
+        while (live == 0 && cbase + 64 < nsteps) {                                   \
+          cbase += 64;                                                               \
+          const int sl = cbase + lane;                                               \
+          chunk_bm = quadrant_blocks(sl < nsteps ? step_g[sl] : 0u, gsh_r, gsh_c);   \
+          live = __ballot(chunk_bm != 0u);                                           \
+        }                                                                            \
+        uint32_t bm = 0;                                                             \
+        int st = 0;                                                                  \
+        if (live) {                                                                  \
+          const int l = __builtin_ctzll(live);                                       \
+          live &= live - 1;                                                          \
+          st = cbase + l;                                                            \
+          bm = __builtin_amdgcn_readlane(chunk_bm, l);                               \
+        }                                                                            \
+        blkmask[S] = bm;                                                             \
+        _Pragma("unroll") for (int g = 0; g < kGroups; ++g) {                        \
+          const int Q = kStepCols * st + 4 * g + kc;                                 \
+          const uint2 ab = col_ab[Q];                                                \
+          if constexpr (kMasked) {                                                   \
+            const uint32_t cm = col_m[Q];                                            \
+            _Pragma("unroll") for (int h = 0; h < 2; ++h) {                          \
+              const uint32_t need = (bm & (3u << (2 * h))) ? rbit[h] : 0u;           \
+              const uint32_t c = (cm & need) ? ab.x : a_zero;                        \
+              ra[S][2 * g + h] = a_lane[(int64_t)c * 64 + 16 * h];                   \
+            }                                                                        \
+            _Pragma("unroll") for (int q = 0; q < 2; ++q) {                          \
+              const uint32_t need = (bm & (5u << q)) ? cbit[q] : 0u;                 \
+              const uint32_t c = (cm & need) ? ab.y : b_zero;                        \
+              rb[S][2 * g + q] = b_lane[(int64_t)c * 64 + 16 * q];                   \
+            }                                                                        \
+          } else {                                                                   \
+            /* past the last live step the loads read step 0 and feed no MFMA */     \
+            const double* pa = a_lane + (int64_t)ab.x * 64;                          \
+            const double* pb = b_lane + (int64_t)ab.y * 64;                          \
+            ra[S][2 * g] = pa[0];                                                    \
+            ra[S][2 * g + 1] = pa[16];                                               \
+            rb[S][2 * g] = pb[0];                                                    \
+            rb[S][2 * g + 1] = pb[16];                                               \
+          }                                                                          \
+        }                                                                            \
+      }
+      // the groups' MFMAs interleave over the four accumulators
+  #define LAMB_MFMA(S)                                                               \
+      {                                                                              \
+        const uint32_t bm = blkmask[S];                                              \
+        _Pragma("unroll") for (int g = 0; g < kGroups; ++g)                          \
+          _Pragma("unroll") for (int h = 0; h < 2; ++h)                              \
+            _Pragma("unroll") for (int q = 0; q < 2; ++q)                            \
+              if ((bm >> (2 * h + q)) & 1u)                                          \
+                acc[2 * h + q] = __builtin_amdgcn_mfma_f64_16x16x4f64(               \
+                    ra[S][2 * g + h], rb[S][2 * g + q], acc[2 * h + q], 0, 0, 0);    \
+      }
+      // slots spelled out so that every register array index is a constant
+  #define LAMB_SLOTS(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
+      static_assert(kPrefetch >= 1 && kPrefetch <= 8, "prefetch depth 1..8");
+  #define LAMB_PRIME(S) if (S < kPrefetch) LAMB_FETCH(S)
+  #define LAMB_ANY(S) if (S < kPrefetch) any |= blkmask[S];
+  #define LAMB_STEP(S) if (S < kPrefetch) { LAMB_MFMA(S); LAMB_FETCH(S); }
+      LAMB_SLOTS(LAMB_PRIME)
+      for (;;) {
+        uint32_t any = 0;
+        LAMB_SLOTS(LAMB_ANY)
+        if (!any) break;
+        LAMB_SLOTS(LAMB_STEP)
+      }
+  #undef LAMB_STEP
+  #undef LAMB_ANY
+  #undef LAMB_PRIME
+  #undef LAMB_SLOTS
+  #undef LAMB_FETCH
+  #undef LAMB_MFMA
+    };
+
+    for (int Q0 = 0; Q0 < ktot; Q0 += kTableCols) {
+      const int nsteps = build_step_table<kStepCols>(T, sm, entries, Q0, col_ab, col_m, step_g,
+                                                     a_zero, b_zero,
+                                          &win_partial, tid);
+      if (win_partial)
+        walk(nsteps, std::true_type{});
+      else
+        walk(nsteps, std::false_type{});
     }
-  } else {
-    numeric_f64_kernel_tile<kLoop>(
-        blockIdx.x, tiles, perm, entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta,
-        eps2, old_row_p, old_col, old_off, old_data, c_data, kept_out, nullptr,
-        nullptr);
+    __syncthreads();
+
+    // ---- epilogue: C tile to LDS ----
+    // v_mfma_f64_16x16x4 result: D[i][j] in lane 16*(i%4) + j, register i/4.
+    double* Ct = lds;
+  #pragma unroll
+    for (int h = 0; h < 2; ++h)
+  #pragma unroll
+      for (int q = 0; q < 2; ++q)
+  #pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = 32 * wr + 16 * h + 4 * r + (lane >> 4);
+          const int col = 32 * wc + 16 * q + (lane & 15);
+          Ct[row * kPitch + col] = acc[2 * h + q][r];
+        }
+    __syncthreads();
+    tile_epilogue<256>(Ct, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data,
+                       c_data, kept_out, tid, kLoop ? slot_counter : nullptr, &next_slot);
+    if constexpr (!kLoop) break;
+    __syncthreads();  // next_slot is visible; the next set-up rewrites the LDS
+    slot = __builtin_amdgcn_readfirstlane(next_slot);
+    if (slot >= n_tiles) break;
   }
 }
 
@@ -420,157 +400,6 @@ __device__ __forceinline__ uint32_t tile_blocks(uint32_t g) {
 // kWhole: every entry of the product covers its strips' atoms (no
 // kEntryPartial), so lanes load whole panel columns without masks.
 template <bool kWhole, bool kLoop>
-__device__ __forceinline__ void numeric_f64_ksplit_kernel_tile(
-    int32_t slot, const TileDesc* __restrict__ tiles, const int32_t* __restrict__ perm,
-    const Entry* __restrict__ entries, const Band* __restrict__ bands,
-    const double* __restrict__ a_pack, const double* __restrict__ b_pack, uint32_t a_zero,
-    uint32_t b_zero, double alpha, double beta, double eps2,
-    const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
-    const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
-    double* __restrict__ c_data, uint16_t* __restrict__ kept_out,
-    int32_t* __restrict__ slot_counter, int32_t* next_slot) {
-  __shared__ double lds[kBandRows * kPitch];
-  uint2* col_ab = reinterpret_cast<uint2*>(lds);
-  uint32_t* col_m = reinterpret_cast<uint32_t*>(col_ab + kTableCols);
-  uint32_t* step_g = col_m + kTableCols;
-  __shared__ TileSmem sm;
-  __shared__ int win_partial;
-
-  const int tid = tile_tid<kLoop>();
-  const int lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const TileInfo T = tile_setup(tiles, perm, bands, sm, slot, tid);
-  // atom mask bit of this lane's row in row block h / column in column block q
-  uint32_t rbit[4] = {0u, 0u, 0u, 0u}, cbit[4] = {0u, 0u, 0u, 0u};
-  for (int a = 0; a < T.natI; ++a)
-#pragma unroll
-    for (int h = 0; h < 4; ++h) {
-      const int r = 16 * h + (lane & 15);
-      if (r >= sm.rs_r[a] && r < sm.rs_r[a + 1]) rbit[h] = 1u << a;
-    }
-  for (int a = 0; a < T.natJ; ++a)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int c = 16 * q + (lane & 15);
-      if (c >= sm.rs_c[a] && c < sm.rs_c[a + 1]) cbit[q] = 1u << (16 + a);
-    }
-  const int kc = lane >> 4;
-  const double* a_lane = a_pack + (lane & 15);
-  const double* b_lane = b_pack + (lane & 15);
-
-  dbl4 acc[16];
-#pragma unroll
-  for (int x = 0; x < 16; ++x) acc[x] = dbl4{0.0, 0.0, 0.0, 0.0};
-  uint32_t seen = 0;  // live steps met so far (wave-uniform)
-
-  for (int Q0 = 0; Q0 < T.ktot; Q0 += kTableCols) {
-    const int nsteps = build_step_table<4>(T, sm, entries, Q0, col_ab, col_m, step_g, a_zero, b_zero,
-                                        &win_partial, tid);
-    int cbase = -64;
-    uint64_t live = 0;
-    uint32_t chunk_bm = 0;
-    double ra[8][4], rb[8][4];
-    uint32_t blkmask[8];
-#define KS_FETCH(S)                                                                \
-    {                                                                              \
-      uint32_t bm = 0;                                                             \
-      int st = 0;                                                                  \
-      while (live == 0 && cbase + 64 < nsteps) {                                   \
-        cbase += 64;                                                               \
-        const int sl = cbase + lane;                                               \
-        chunk_bm = tile_blocks(sl < nsteps ? step_g[sl] : 0u);                     \
-        /* this wave's live steps: those whose rank is wave mod 4 */               \
-        const uint64_t any = __ballot(chunk_bm != 0u);                             \
-        const uint32_t rank = seen + __builtin_amdgcn_mbcnt_hi(                    \
-            (uint32_t)(any >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)any, 0u));  \
-        live = __ballot(chunk_bm != 0u && (rank & 3u) == (uint32_t)wave);          \
-        seen += (uint32_t)__builtin_popcountll(any);                               \
-      }                                                                            \
-      if (live) {                                                                  \
-        const int l = __builtin_ctzll(live);                                       \
-        live &= live - 1;                                                          \
-        st = cbase + l;                                                            \
-        bm = __builtin_amdgcn_readlane(chunk_bm, l);                               \
-      }                                                                            \
-      blkmask[S] = bm;                                                             \
-      const int Q = 4 * st + kc;                                                   \
-      const uint2 ab = col_ab[Q];                                                  \
-      if constexpr (kWhole) {                                                      \
-        /* without a step of this wave the loads read step 0, feeding no MFMA */   \
-        const double* pa = a_lane + (int64_t)ab.x * 64;                            \
-        const double* pb = b_lane + (int64_t)ab.y * 64;                            \
-        _Pragma("unroll") for (int h = 0; h < 4; ++h) ra[S][h] = pa[16 * h];       \
-        _Pragma("unroll") for (int q = 0; q < 4; ++q) rb[S][q] = pb[16 * q];       \
-        /* keeps the scheduler from hoisting these loads over the MFMAs, */        \
-        /* which spilled 100+ VGPRs */                                             \
-        __builtin_amdgcn_sched_barrier(0);                                         \
-      } else {                                                                     \
-        const uint32_t cm = col_m[Q];                                              \
-        _Pragma("unroll") for (int h = 0; h < 4; ++h) {                            \
-          const uint32_t need = (bm & (0xFu << (4 * h))) ? rbit[h] : 0u;           \
-          const uint32_t c = (cm & need) ? ab.x : a_zero;                          \
-          ra[S][h] = a_lane[(int64_t)c * 64 + 16 * h];                             \
-        }                                                                          \
-        _Pragma("unroll") for (int q = 0; q < 4; ++q) {                            \
-          const uint32_t need = (bm & (0x1111u << q)) ? cbit[q] : 0u;              \
-          const uint32_t c = (cm & need) ? ab.y : b_zero;                          \
-          rb[S][q] = b_lane[(int64_t)c * 64 + 16 * q];                             \
-        }                                                                          \
-      }                                                                            \
-    }
-#define KS_MFMA(S)                                                                 \
-    {                                                                              \
-      const uint32_t bm = blkmask[S];                                              \
-      _Pragma("unroll") for (int h = 0; h < 4; ++h)                                \
-        _Pragma("unroll") for (int q = 0; q < 4; ++q)                              \
-          if ((bm >> (4 * h + q)) & 1u)                                            \
-            acc[4 * h + q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[S][h], rb[S][q], acc[4 * h + q], 0, 0, 0); \
-    }
-#define KS_SLOTS(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
-    static_assert(kKsPrefetch >= 1 && kKsPrefetch <= 8, "prefetch depth 1..8");
-#define KS_PRIME(S) if (S < kKsPrefetch) KS_FETCH(S)
-#define KS_ANY(S) if (S < kKsPrefetch) any |= blkmask[S];
-#define KS_STEP(S) if (S < kKsPrefetch) { KS_MFMA(S); KS_FETCH(S); }
-    KS_SLOTS(KS_PRIME)
-    for (;;) {
-      uint32_t any = 0;
-      KS_SLOTS(KS_ANY)
-      if (!any) break;
-      KS_SLOTS(KS_STEP)
-    }
-#undef KS_STEP
-#undef KS_ANY
-#undef KS_PRIME
-#undef KS_SLOTS
-#undef KS_FETCH
-#undef KS_MFMA
-  }
-  __syncthreads();
-
-  // ---- epilogue: the four partial tiles summed into LDS in wave order ----
-  // v_mfma_f64_16x16x4 result: D[i][j] in lane 16*(i%4) + j, register i/4.
-  double* Ct = lds;
-  for (int w = 0; w < 4; ++w) {
-    if (wave == w) {
-#pragma unroll
-      for (int h = 0; h < 4; ++h)
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int idx = (16 * h + 4 * r + (lane >> 4)) * kPitch + 16 * q + (lane & 15);
-            const double v = acc[4 * h + q][r];
-            Ct[idx] = w == 0 ? v : Ct[idx] + v;
-          }
-    }
-    __syncthreads();
-  }
-  tile_epilogue<256>(Ct, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data,
-                     c_data, kept_out, tid, slot_counter,
-                     next_slot);
-}
-
-template <bool kWhole, bool kLoop>
 __global__ __launch_bounds__(256, 2) void numeric_f64_ksplit_kernel(
     int32_t n_tiles, int32_t* __restrict__ slot_counter, const TileDesc* __restrict__ tiles,
     const int32_t* __restrict__ perm,
@@ -581,23 +410,154 @@ __global__ __launch_bounds__(256, 2) void numeric_f64_ksplit_kernel(
     const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
     double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
   LAMB_GUARD;
-  if constexpr (kLoop) {
-    // slots: blockIdx.x first, then the counter's (set to the grid size
-    // before the launch), claimed during each tile's epilogue
-    __shared__ int32_t next_slot;
-    for (int32_t slot = blockIdx.x; slot < n_tiles;) {
-      numeric_f64_ksplit_kernel_tile<kWhole, kLoop>(
-          slot, tiles, perm, entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta,
-          eps2, old_row_p, old_col, old_off, old_data, c_data, kept_out, slot_counter,
-          &next_slot);
-      __syncthreads();  // next_slot is visible; the next set-up rewrites the LDS
-      slot = __builtin_amdgcn_readfirstlane(next_slot);
+  // slots: blockIdx.x first; with kLoop (a persistent grid) the next ones
+  // from the counter (set to the grid size before the launch), claimed
+  // during each tile's epilogue. Without kLoop the body runs once, as a
+  // plain one-tile kernel with that kernel's registers.
+  __shared__ int32_t next_slot;
+  for (int32_t slot = blockIdx.x;;) {
+    __shared__ double lds[kBandRows * kPitch];
+    uint2* col_ab = reinterpret_cast<uint2*>(lds);
+    uint32_t* col_m = reinterpret_cast<uint32_t*>(col_ab + kTableCols);
+    uint32_t* step_g = col_m + kTableCols;
+    __shared__ TileSmem sm;
+    __shared__ int win_partial;
+
+    const int tid = tile_tid<kLoop>();
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const TileInfo T = tile_setup(tiles, perm, bands, sm, slot, tid);
+    // atom mask bit of this lane's row in row block h / column in column block q
+    uint32_t rbit[4] = {0u, 0u, 0u, 0u}, cbit[4] = {0u, 0u, 0u, 0u};
+    for (int a = 0; a < T.natI; ++a)
+  #pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        const int r = 16 * h + (lane & 15);
+        if (r >= sm.rs_r[a] && r < sm.rs_r[a + 1]) rbit[h] = 1u << a;
+      }
+    for (int a = 0; a < T.natJ; ++a)
+  #pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int c = 16 * q + (lane & 15);
+        if (c >= sm.rs_c[a] && c < sm.rs_c[a + 1]) cbit[q] = 1u << (16 + a);
+      }
+    const int kc = lane >> 4;
+    const double* a_lane = a_pack + (lane & 15);
+    const double* b_lane = b_pack + (lane & 15);
+
+    dbl4 acc[16];
+  #pragma unroll
+    for (int x = 0; x < 16; ++x) acc[x] = dbl4{0.0, 0.0, 0.0, 0.0};
+    uint32_t seen = 0;  // live steps met so far (wave-uniform)
+
+    for (int Q0 = 0; Q0 < T.ktot; Q0 += kTableCols) {
+      const int nsteps = build_step_table<4>(T, sm, entries, Q0, col_ab, col_m, step_g, a_zero, b_zero,
+                                          &win_partial, tid);
+      int cbase = -64;
+      uint64_t live = 0;
+      uint32_t chunk_bm = 0;
+      double ra[8][4], rb[8][4];
+      uint32_t blkmask[8];
+  #define KS_FETCH(S)                                                                \
+      {                                                                              \
+        uint32_t bm = 0;                                                             \
+        int st = 0;                                                                  \
+        while (live == 0 && cbase + 64 < nsteps) {                                   \
+          cbase += 64;                                                               \
+          const int sl = cbase + lane;                                               \
+          chunk_bm = tile_blocks(sl < nsteps ? step_g[sl] : 0u);                     \
+          /* this wave's live steps: those whose rank is wave mod 4 */               \
+          const uint64_t any = __ballot(chunk_bm != 0u);                             \
+          const uint32_t rank = seen + __builtin_amdgcn_mbcnt_hi(                    \
+              (uint32_t)(any >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)any, 0u));  \
+          live = __ballot(chunk_bm != 0u && (rank & 3u) == (uint32_t)wave);          \
+          seen += (uint32_t)__builtin_popcountll(any);                               \
+        }                                                                            \
+        if (live) {                                                                  \
+          const int l = __builtin_ctzll(live);                                       \
+          live &= live - 1;                                                          \
+          st = cbase + l;                                                            \
+          bm = __builtin_amdgcn_readlane(chunk_bm, l);                               \
+        }                                                                            \
+        blkmask[S] = bm;                                                             \
+        const int Q = 4 * st + kc;                                                   \
+        const uint2 ab = col_ab[Q];                                                  \
+        if constexpr (kWhole) {                                                      \
+          /* without a step of this wave the loads read step 0, feeding no MFMA */   \
+          const double* pa = a_lane + (int64_t)ab.x * 64;                            \
+          const double* pb = b_lane + (int64_t)ab.y * 64;                            \
+          _Pragma("unroll") for (int h = 0; h < 4; ++h) ra[S][h] = pa[16 * h];       \
+          _Pragma("unroll") for (int q = 0; q < 4; ++q) rb[S][q] = pb[16 * q];       \
+          /* keeps the scheduler from hoisting these loads over the MFMAs, */        \
+          /* which spilled 100+ VGPRs */                                             \
+          __builtin_amdgcn_sched_barrier(0);                                         \
+        } else {                                                                     \
+          const uint32_t cm = col_m[Q];                                              \
+          _Pragma("unroll") for (int h = 0; h < 4; ++h) {                            \
+            const uint32_t need = (bm & (0xFu << (4 * h))) ? rbit[h] : 0u;           \
+            const uint32_t c = (cm & need) ? ab.x : a_zero;                          \
+            ra[S][h] = a_lane[(int64_t)c * 64 + 16 * h];                             \
+          }                                                                          \
+          _Pragma("unroll") for (int q = 0; q < 4; ++q) {                            \
+            const uint32_t need = (bm & (0x1111u << q)) ? cbit[q] : 0u;              \
+            const uint32_t c = (cm & need) ? ab.y : b_zero;                          \
+            rb[S][q] = b_lane[(int64_t)c * 64 + 16 * q];                             \
+          }                                                                          \
+        }                                                                            \
+      }
+  #define KS_MFMA(S)                                                                 \
+      {                                                                              \
+        const uint32_t bm = blkmask[S];                                              \
+        _Pragma("unroll") for (int h = 0; h < 4; ++h)                                \
+          _Pragma("unroll") for (int q = 0; q < 4; ++q)                              \
+            if ((bm >> (4 * h + q)) & 1u)                                            \
+              acc[4 * h + q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[S][h], rb[S][q], acc[4 * h + q], 0, 0, 0); \
+      }
+  #define KS_SLOTS(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
+      static_assert(kKsPrefetch >= 1 && kKsPrefetch <= 8, "prefetch depth 1..8");
+  #define KS_PRIME(S) if (S < kKsPrefetch) KS_FETCH(S)
+  #define KS_ANY(S) if (S < kKsPrefetch) any |= blkmask[S];
+  #define KS_STEP(S) if (S < kKsPrefetch) { KS_MFMA(S); KS_FETCH(S); }
+      KS_SLOTS(KS_PRIME)
+      for (;;) {
+        uint32_t any = 0;
+        KS_SLOTS(KS_ANY)
+        if (!any) break;
+        KS_SLOTS(KS_STEP)
+      }
+  #undef KS_STEP
+  #undef KS_ANY
+  #undef KS_PRIME
+  #undef KS_SLOTS
+  #undef KS_FETCH
+  #undef KS_MFMA
     }
-  } else {
-    numeric_f64_ksplit_kernel_tile<kWhole, kLoop>(
-        blockIdx.x, tiles, perm, entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta,
-        eps2, old_row_p, old_col, old_off, old_data, c_data, kept_out, nullptr,
-        nullptr);
+    __syncthreads();
+
+    // ---- epilogue: the four partial tiles summed into LDS in wave order ----
+    // v_mfma_f64_16x16x4 result: D[i][j] in lane 16*(i%4) + j, register i/4.
+    double* Ct = lds;
+    for (int w = 0; w < 4; ++w) {
+      if (wave == w) {
+  #pragma unroll
+        for (int h = 0; h < 4; ++h)
+  #pragma unroll
+          for (int q = 0; q < 4; ++q)
+  #pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int idx = (16 * h + 4 * r + (lane >> 4)) * kPitch + 16 * q + (lane & 15);
+              const double v = acc[4 * h + q][r];
+              Ct[idx] = w == 0 ? v : Ct[idx] + v;
+            }
+      }
+      __syncthreads();
+    }
+    tile_epilogue<256>(Ct, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data,
+                       c_data, kept_out, tid, kLoop ? slot_counter : nullptr, &next_slot);
+    if constexpr (!kLoop) break;
+    __syncthreads();  // next_slot is visible; the next set-up rewrites the LDS
+    slot = __builtin_amdgcn_readfirstlane(next_slot);
+    if (slot >= n_tiles) break;
   }
 }
 
@@ -616,163 +576,6 @@ __global__ __launch_bounds__(256, 2) void numeric_f64_ksplit_kernel(
 constexpr int kHalfPrefetch = LAMB_HALF_PREFETCH;
 
 template <bool kWhole, bool kLoop>
-__device__ __forceinline__ void numeric_f64_half_kernel_tile(
-    int32_t slot, const TileDesc* __restrict__ tiles, const int32_t* __restrict__ perm,
-    const Entry* __restrict__ entries, const Band* __restrict__ bands,
-    const double* __restrict__ a_pack, const double* __restrict__ b_pack, uint32_t a_zero,
-    uint32_t b_zero, double alpha, double beta, double eps2,
-    const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
-    const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
-    double* __restrict__ c_data, uint16_t* __restrict__ kept_out,
-    int32_t* __restrict__ slot_counter, int32_t* next_slot) {
-  __shared__ double lds[kBandRows * kPitch];
-  uint2* col_ab = reinterpret_cast<uint2*>(lds);
-  uint32_t* col_m = reinterpret_cast<uint32_t*>(col_ab + kTableCols);
-  uint32_t* step_g = col_m + kTableCols;
-  __shared__ TileSmem sm;
-  __shared__ int win_partial;
-
-  const int tid = tile_tid<kLoop>();
-  const int lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int wc = wave & 1;
-  const uint32_t par = (uint32_t)(wave >> 1);
-  const TileInfo T = tile_setup(tiles, perm, bands, sm, slot, tid);
-  // atom mask bit of this lane's row in row block h / column in column block q
-  uint32_t rbit[4] = {0u, 0u, 0u, 0u}, cbit[2] = {0u, 0u};
-  for (int a = 0; a < T.natI; ++a)
-#pragma unroll
-    for (int h = 0; h < 4; ++h) {
-      const int r = 16 * h + (lane & 15);
-      if (r >= sm.rs_r[a] && r < sm.rs_r[a + 1]) rbit[h] = 1u << a;
-    }
-  for (int a = 0; a < T.natJ; ++a)
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int c = 32 * wc + 16 * q + (lane & 15);
-      if (c >= sm.rs_c[a] && c < sm.rs_c[a + 1]) cbit[q] = 1u << (16 + a);
-    }
-  const int kc = lane >> 4;
-  const double* a_lane = a_pack + (lane & 15);
-  const double* b_lane = b_pack + 32 * wc + (lane & 15);
-  // bits 4h + 2wc + q of a tile block mask: this wave's half
-  const uint32_t half_bits = 0x3333u << (2 * wc);
-
-  dbl4 acc[8];
-#pragma unroll
-  for (int x = 0; x < 8; ++x) acc[x] = dbl4{0.0, 0.0, 0.0, 0.0};
-  uint32_t seen = 0;  // live steps of this half met so far (wave-uniform)
-
-  for (int Q0 = 0; Q0 < T.ktot; Q0 += kTableCols) {
-    const int nsteps = build_step_table<4>(T, sm, entries, Q0, col_ab, col_m, step_g, a_zero,
-                                           b_zero, &win_partial, tid);
-    int cbase = -64;
-    uint64_t live = 0;
-    uint32_t chunk_bm = 0;
-    double ra[8][4], rb[8][2];
-    uint32_t blkmask[8];
-#define HF_FETCH(S)                                                                \
-    {                                                                              \
-      uint32_t bm = 0;                                                             \
-      int st = 0;                                                                  \
-      while (live == 0 && cbase + 64 < nsteps) {                                   \
-        cbase += 64;                                                               \
-        const int sl = cbase + lane;                                               \
-        chunk_bm = tile_blocks(sl < nsteps ? step_g[sl] : 0u) & half_bits;         \
-        /* this wave's live steps: those of rank parity par in the half */         \
-        const uint64_t any = __ballot(chunk_bm != 0u);                             \
-        const uint32_t rank = seen + __builtin_amdgcn_mbcnt_hi(                    \
-            (uint32_t)(any >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)any, 0u));  \
-        live = __ballot(chunk_bm != 0u && (rank & 1u) == par);                     \
-        seen += (uint32_t)__builtin_popcountll(any);                               \
-      }                                                                            \
-      if (live) {                                                                  \
-        const int l = __builtin_ctzll(live);                                       \
-        live &= live - 1;                                                          \
-        st = cbase + l;                                                            \
-        bm = __builtin_amdgcn_readlane(chunk_bm, l);                               \
-      }                                                                            \
-      blkmask[S] = bm;                                                             \
-      const int Q = 4 * st + kc;                                                   \
-      const uint2 ab = col_ab[Q];                                                  \
-      if constexpr (kWhole) {                                                      \
-        /* without a step of this wave the loads read step 0, feeding no MFMA */   \
-        const double* pa = a_lane + (int64_t)ab.x * 64;                            \
-        const double* pb = b_lane + (int64_t)ab.y * 64;                            \
-        _Pragma("unroll") for (int h = 0; h < 4; ++h) ra[S][h] = pa[16 * h];       \
-        _Pragma("unroll") for (int q = 0; q < 2; ++q) rb[S][q] = pb[16 * q];       \
-        __builtin_amdgcn_sched_barrier(0);                                         \
-      } else {                                                                     \
-        const uint32_t cm = col_m[Q];                                              \
-        _Pragma("unroll") for (int h = 0; h < 4; ++h) {                            \
-          const uint32_t need = (bm & (0xFu << (4 * h))) ? rbit[h] : 0u;           \
-          const uint32_t c = (cm & need) ? ab.x : a_zero;                          \
-          ra[S][h] = a_lane[(int64_t)c * 64 + 16 * h];                             \
-        }                                                                          \
-        _Pragma("unroll") for (int q = 0; q < 2; ++q) {                            \
-          const uint32_t need = (bm & (0x1111u << (2 * wc + q))) ? cbit[q] : 0u;   \
-          const uint32_t c = (cm & need) ? ab.y : b_zero;                          \
-          rb[S][q] = b_lane[(int64_t)c * 64 + 16 * q];                             \
-        }                                                                          \
-      }                                                                            \
-    }
-#define HF_MFMA(S)                                                                 \
-    {                                                                              \
-      const uint32_t bm = blkmask[S] >> (2 * wc);                                  \
-      _Pragma("unroll") for (int h = 0; h < 4; ++h)                                \
-        _Pragma("unroll") for (int q = 0; q < 2; ++q)                              \
-          if ((bm >> (4 * h + q)) & 1u)                                            \
-            acc[2 * h + q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[S][h], rb[S][q], acc[2 * h + q], 0, 0, 0); \
-    }
-#define HF_SLOTS(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
-    // the masked walk holds per-lane atom masks besides the ring: two steps
-    // in flight keep it within the registers of four waves per SIMD
-    constexpr int kDepth = kWhole ? kHalfPrefetch : (kHalfPrefetch < 2 ? kHalfPrefetch : 2);
-    static_assert(kHalfPrefetch >= 1 && kHalfPrefetch <= 8, "prefetch depth 1..8");
-#define HF_PRIME(S) if (S < kDepth) HF_FETCH(S)
-#define HF_ANY(S) if (S < kDepth) any |= blkmask[S];
-#define HF_STEP(S) if (S < kDepth) { HF_MFMA(S); HF_FETCH(S); }
-    HF_SLOTS(HF_PRIME)
-    for (;;) {
-      uint32_t any = 0;
-      HF_SLOTS(HF_ANY)
-      if (!any) break;
-      HF_SLOTS(HF_STEP)
-    }
-#undef HF_STEP
-#undef HF_ANY
-#undef HF_PRIME
-#undef HF_SLOTS
-#undef HF_FETCH
-#undef HF_MFMA
-  }
-  __syncthreads();
-
-  // ---- epilogue: the two partial halves added in LDS ----
-  // v_mfma_f64_16x16x4 result: D[i][j] in lane 16*(i%4) + j, register i/4.
-  double* Ct = lds;
-  for (int p = 0; p < 2; ++p) {
-    if ((wave >> 1) == p) {
-#pragma unroll
-      for (int h = 0; h < 4; ++h)
-#pragma unroll
-        for (int q = 0; q < 2; ++q)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int idx =
-                (16 * h + 4 * r + (lane >> 4)) * kPitch + 32 * wc + 16 * q + (lane & 15);
-            const double v = acc[2 * h + q][r];
-            Ct[idx] = p == 0 ? v : Ct[idx] + v;
-          }
-    }
-    __syncthreads();
-  }
-  tile_epilogue<256>(Ct, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data,
-                     c_data, kept_out, tid, slot_counter,
-                     next_slot);
-}
-
-template <bool kWhole, bool kLoop>
 __global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
     int32_t n_tiles, int32_t* __restrict__ slot_counter, const TileDesc* __restrict__ tiles,
     const int32_t* __restrict__ perm,
@@ -783,23 +586,160 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
     const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
     double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
   LAMB_GUARD;
-  if constexpr (kLoop) {
-    // slots: blockIdx.x first, then the counter's (set to the grid size
-    // before the launch), claimed during each tile's epilogue
-    __shared__ int32_t next_slot;
-    for (int32_t slot = blockIdx.x; slot < n_tiles;) {
-      numeric_f64_half_kernel_tile<kWhole, kLoop>(
-          slot, tiles, perm, entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta,
-          eps2, old_row_p, old_col, old_off, old_data, c_data, kept_out, slot_counter,
-          &next_slot);
-      __syncthreads();  // next_slot is visible; the next set-up rewrites the LDS
-      slot = __builtin_amdgcn_readfirstlane(next_slot);
+  // slots: blockIdx.x first; with kLoop (a persistent grid) the next ones
+  // from the counter (set to the grid size before the launch), claimed
+  // during each tile's epilogue. Without kLoop the body runs once, as a
+  // plain one-tile kernel with that kernel's registers.
+  __shared__ int32_t next_slot;
+  for (int32_t slot = blockIdx.x;;) {
+    __shared__ double lds[kBandRows * kPitch];
+    uint2* col_ab = reinterpret_cast<uint2*>(lds);
+    uint32_t* col_m = reinterpret_cast<uint32_t*>(col_ab + kTableCols);
+    uint32_t* step_g = col_m + kTableCols;
+    __shared__ TileSmem sm;
+    __shared__ int win_partial;
+
+    const int tid = tile_tid<kLoop>();
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wc = wave & 1;
+    const uint32_t par = (uint32_t)(wave >> 1);
+    const TileInfo T = tile_setup(tiles, perm, bands, sm, slot, tid);
+    // atom mask bit of this lane's row in row block h / column in column block q
+    uint32_t rbit[4] = {0u, 0u, 0u, 0u}, cbit[2] = {0u, 0u};
+    for (int a = 0; a < T.natI; ++a)
+  #pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        const int r = 16 * h + (lane & 15);
+        if (r >= sm.rs_r[a] && r < sm.rs_r[a + 1]) rbit[h] = 1u << a;
+      }
+    for (int a = 0; a < T.natJ; ++a)
+  #pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int c = 32 * wc + 16 * q + (lane & 15);
+        if (c >= sm.rs_c[a] && c < sm.rs_c[a + 1]) cbit[q] = 1u << (16 + a);
+      }
+    const int kc = lane >> 4;
+    const double* a_lane = a_pack + (lane & 15);
+    const double* b_lane = b_pack + 32 * wc + (lane & 15);
+    // bits 4h + 2wc + q of a tile block mask: this wave's half
+    const uint32_t half_bits = 0x3333u << (2 * wc);
+
+    dbl4 acc[8];
+  #pragma unroll
+    for (int x = 0; x < 8; ++x) acc[x] = dbl4{0.0, 0.0, 0.0, 0.0};
+    uint32_t seen = 0;  // live steps of this half met so far (wave-uniform)
+
+    for (int Q0 = 0; Q0 < T.ktot; Q0 += kTableCols) {
+      const int nsteps = build_step_table<4>(T, sm, entries, Q0, col_ab, col_m, step_g, a_zero,
+                                             b_zero, &win_partial, tid);
+      int cbase = -64;
+      uint64_t live = 0;
+      uint32_t chunk_bm = 0;
+      double ra[8][4], rb[8][2];
+      uint32_t blkmask[8];
+  #define HF_FETCH(S)                                                                \
+      {                                                                              \
+        uint32_t bm = 0;                                                             \
+        int st = 0;                                                                  \
+        while (live == 0 && cbase + 64 < nsteps) {                                   \
+          cbase += 64;                                                               \
+          const int sl = cbase + lane;                                               \
+          chunk_bm = tile_blocks(sl < nsteps ? step_g[sl] : 0u) & half_bits;         \
+          /* this wave's live steps: those of rank parity par in the half */         \
+          const uint64_t any = __ballot(chunk_bm != 0u);                             \
+          const uint32_t rank = seen + __builtin_amdgcn_mbcnt_hi(                    \
+              (uint32_t)(any >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)any, 0u));  \
+          live = __ballot(chunk_bm != 0u && (rank & 1u) == par);                     \
+          seen += (uint32_t)__builtin_popcountll(any);                               \
+        }                                                                            \
+        if (live) {                                                                  \
+          const int l = __builtin_ctzll(live);                                       \
+          live &= live - 1;                                                          \
+          st = cbase + l;                                                            \
+          bm = __builtin_amdgcn_readlane(chunk_bm, l);                               \
+        }                                                                            \
+        blkmask[S] = bm;                                                             \
+        const int Q = 4 * st + kc;                                                   \
+        const uint2 ab = col_ab[Q];                                                  \
+        if constexpr (kWhole) {                                                      \
+          /* without a step of this wave the loads read step 0, feeding no MFMA */   \
+          const double* pa = a_lane + (int64_t)ab.x * 64;                            \
+          const double* pb = b_lane + (int64_t)ab.y * 64;                            \
+          _Pragma("unroll") for (int h = 0; h < 4; ++h) ra[S][h] = pa[16 * h];       \
+          _Pragma("unroll") for (int q = 0; q < 2; ++q) rb[S][q] = pb[16 * q];       \
+          __builtin_amdgcn_sched_barrier(0);                                         \
+        } else {                                                                     \
+          const uint32_t cm = col_m[Q];                                              \
+          _Pragma("unroll") for (int h = 0; h < 4; ++h) {                            \
+            const uint32_t need = (bm & (0xFu << (4 * h))) ? rbit[h] : 0u;           \
+            const uint32_t c = (cm & need) ? ab.x : a_zero;                          \
+            ra[S][h] = a_lane[(int64_t)c * 64 + 16 * h];                             \
+          }                                                                          \
+          _Pragma("unroll") for (int q = 0; q < 2; ++q) {                            \
+            const uint32_t need = (bm & (0x1111u << (2 * wc + q))) ? cbit[q] : 0u;   \
+            const uint32_t c = (cm & need) ? ab.y : b_zero;                          \
+            rb[S][q] = b_lane[(int64_t)c * 64 + 16 * q];                             \
+          }                                                                          \
+        }                                                                            \
+      }
+  #define HF_MFMA(S)                                                                 \
+      {                                                                              \
+        const uint32_t bm = blkmask[S] >> (2 * wc);                                  \
+        _Pragma("unroll") for (int h = 0; h < 4; ++h)                                \
+          _Pragma("unroll") for (int q = 0; q < 2; ++q)                              \
+            if ((bm >> (4 * h + q)) & 1u)                                            \
+              acc[2 * h + q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[S][h], rb[S][q], acc[2 * h + q], 0, 0, 0); \
+      }
+  #define HF_SLOTS(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
+      // the masked walk holds per-lane atom masks besides the ring: two steps
+      // in flight keep it within the registers of four waves per SIMD
+      constexpr int kDepth = kWhole ? kHalfPrefetch : (kHalfPrefetch < 2 ? kHalfPrefetch : 2);
+      static_assert(kHalfPrefetch >= 1 && kHalfPrefetch <= 8, "prefetch depth 1..8");
+  #define HF_PRIME(S) if (S < kDepth) HF_FETCH(S)
+  #define HF_ANY(S) if (S < kDepth) any |= blkmask[S];
+  #define HF_STEP(S) if (S < kDepth) { HF_MFMA(S); HF_FETCH(S); }
+      HF_SLOTS(HF_PRIME)
+      for (;;) {
+        uint32_t any = 0;
+        HF_SLOTS(HF_ANY)
+        if (!any) break;
+        HF_SLOTS(HF_STEP)
+      }
+  #undef HF_STEP
+  #undef HF_ANY
+  #undef HF_PRIME
+  #undef HF_SLOTS
+  #undef HF_FETCH
+  #undef HF_MFMA
     }
-  } else {
-    numeric_f64_half_kernel_tile<kWhole, kLoop>(
-        blockIdx.x, tiles, perm, entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta,
-        eps2, old_row_p, old_col, old_off, old_data, c_data, kept_out, nullptr,
-        nullptr);
+    __syncthreads();
+
+    // ---- epilogue: the two partial halves added in LDS ----
+    // v_mfma_f64_16x16x4 result: D[i][j] in lane 16*(i%4) + j, register i/4.
+    double* Ct = lds;
+    for (int p = 0; p < 2; ++p) {
+      if ((wave >> 1) == p) {
+  #pragma unroll
+        for (int h = 0; h < 4; ++h)
+  #pragma unroll
+          for (int q = 0; q < 2; ++q)
+  #pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int idx =
+                  (16 * h + 4 * r + (lane >> 4)) * kPitch + 32 * wc + 16 * q + (lane & 15);
+              const double v = acc[2 * h + q][r];
+              Ct[idx] = p == 0 ? v : Ct[idx] + v;
+            }
+      }
+      __syncthreads();
+    }
+    tile_epilogue<256>(Ct, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data,
+                       c_data, kept_out, tid, kLoop ? slot_counter : nullptr, &next_slot);
+    if constexpr (!kLoop) break;
+    __syncthreads();  // next_slot is visible; the next set-up rewrites the LDS
+    slot = __builtin_amdgcn_readfirstlane(next_slot);
+    if (slot >= n_tiles) break;
   }
 }
 

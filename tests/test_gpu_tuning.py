@@ -80,7 +80,15 @@ def test_shape_tuning_and_table_round_trip(tmp_path):
         second.close()
 
 
-def test_malformed_tables_refused(ctx, tmp_path):
+def test_malformed_tables_refused(tmp_path):
+    ctx = D.Context(0)  # an empty tuning table
+    try:
+        check_refusals(ctx, tmp_path)
+    finally:
+        ctx.close()
+
+
+def check_refusals(ctx, tmp_path):
     bad = {
         "header": "not a table\nshape 0 0 0 0 0 2\n",
         "variant": "lamb_hip tuning table 1\nshape 0 1 2 3 4 7\n",
@@ -89,6 +97,8 @@ def test_malformed_tables_refused(ctx, tmp_path):
         "fields": "lamb_hip tuning table 1\nshape 0 1 2 3 2\n",
         "trailing": "lamb_hip tuning table 1\norder 0 1 2 3 8 0 9\n",
         "empty": "",
+        # a good entry before a bad one: the whole table is refused
+        "partly": "lamb_hip tuning table 1\norder 0 1 2 3 8 0\norder 0 1 2 x 8 0\n",
     }
     for name, text in bad.items():
         p = tmp_path / f"{name}.txt"
