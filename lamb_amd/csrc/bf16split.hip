@@ -191,10 +191,12 @@ __global__ __launch_bounds__(256, 2) void numeric_bf16_kernel(
     const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
     double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
   LAMB_GUARD;
-  // slots: blockIdx.x first; with kLoop (a persistent grid) the next ones
-  // from the counter (set to the grid size before the launch), claimed
-  // during each tile's epilogue. Without kLoop the body runs once, as a
-  // plain one-tile kernel with that kernel's registers.
+  // slots: blockIdx.x first. With kLoop (a persistent grid) a workgroup then
+  // takes slots x, x + 8, x + 16, ... of its residue x = blockIdx.x mod 8 (the
+  // XCD the hardware deals it to) from counter x, one claimed during each
+  // tile's epilogue: each XCD walks the same tiles, in the same order, as
+  // with one workgroup per tile. Without kLoop the body runs once, as a plain
+  // one-tile kernel with that kernel's registers.
   __shared__ int32_t next_slot;
   for (int32_t slot = blockIdx.x;;) {
     __shared__ double lds[kBandRows * kPitch];
@@ -359,10 +361,11 @@ __global__ __launch_bounds__(256, 2) void numeric_bf16_kernel(
     flush();
     __syncthreads();
     tile_epilogue<256>(ctile, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data,
-                       c_data, kept_out, tid, kLoop ? slot_counter : nullptr, &next_slot);
+                       c_data, kept_out, tid,
+                       kLoop ? slot_counter + (blockIdx.x & 7) : nullptr, &next_slot);
     if constexpr (!kLoop) break;
     __syncthreads();  // next_slot is visible; the next set-up rewrites the LDS
-    slot = __builtin_amdgcn_readfirstlane(next_slot);
+    slot = (int32_t)(blockIdx.x & 7) + 8 * __builtin_amdgcn_readfirstlane(next_slot);
     if (slot >= n_tiles) break;
   }
 }

@@ -114,9 +114,9 @@ hipError_t launch_block_nsq(hipStream_t s, int64_t nblk, const int32_t* d_rows,
 // numeric kernel runs tile slots blockIdx.x, blockIdx.x + gridDim.x, ... of
 // the tile order: persistent = false launches one workgroup per tile,
 // persistent = true the device's resident workgroups, each looping over its
-// slots (the first is blockIdx.x, later ones come from *slot_counter, a
-// device int32 the launch resets); both compute every tile the same way
-// (same bits). grid_used gets the grid launched.
+// slots (the first is blockIdx.x, later ones from slot_counter[blockIdx.x
+// mod 8], eight device int32 the launch resets); both compute every tile the
+// same way (same bits). grid_used gets the grid launched.
 constexpr int kF64Quadrant = 0;
 constexpr int kF64Ksplit = 1;
 constexpr int kF64Half = 2;
@@ -128,8 +128,9 @@ hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* ti
                               const int64_t* old_off, const double* old_data, double* c_data,
                               uint16_t* kept, int variant, bool whole, bool persistent,
                               int32_t* slot_counter, int32_t* grid_used);
-// Enqueues *counter = first (the slot counter of a persistent grid).
-void launch_slot_counter_reset(hipStream_t s, int32_t* counter, int32_t first);
+// Enqueues the reset of a persistent grid's eight slot counters (one per
+// residue of blockIdx.x mod 8) for a grid of `grid` workgroups.
+void launch_slot_counter_reset(hipStream_t s, int32_t* counter, int32_t grid);
 // Workgroups of a 256-thread kernel resident on the current device at once
 // (the grid of a persistent launch; 0 when unknown).
 int32_t resident_workgroups(const void* kernel);

@@ -82,7 +82,7 @@ struct lamb_hip_ctx {
   lamb::DevBuf d_kept, d_apack, d_bpack;
   lamb::DevBuf d_aexp, d_bexp;  // BF16 split: exponent per A row / B column
   lamb::DevBuf d_aslice, d_bslice;  // BF16 split: slice packs
-  lamb::DevBuf d_slot_counter;      // persistent numeric grids: next tile slot
+  lamb::DevBuf d_slot_counter;      // persistent numeric grids: slot counters (8)
   void* sym = nullptr;  // lamb::SymState*
   // autotuner: tile order (row bands per block) and launch form (one
   // workgroup per tile, or the resident workgroups looping over tiles) per

@@ -210,10 +210,12 @@ __global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
     const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
     double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
   LAMB_GUARD;
-  // slots: blockIdx.x first; with kLoop (a persistent grid) the next ones
-  // from the counter (set to the grid size before the launch), claimed
-  // during each tile's epilogue. Without kLoop the body runs once, as a
-  // plain one-tile kernel with that kernel's registers.
+  // slots: blockIdx.x first. With kLoop (a persistent grid) a workgroup then
+  // takes slots x, x + 8, x + 16, ... of its residue x = blockIdx.x mod 8 (the
+  // XCD the hardware deals it to) from counter x, one claimed during each
+  // tile's epilogue: each XCD walks the same tiles, in the same order, as
+  // with one workgroup per tile. Without kLoop the body runs once, as a plain
+  // one-tile kernel with that kernel's registers.
   __shared__ int32_t next_slot;
   for (int32_t slot = blockIdx.x;;) {
     // main loop: the step table of a window of K columns; epilogue: C tile
@@ -359,10 +361,11 @@ __global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
         }
     __syncthreads();
     tile_epilogue<256>(Ct, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data,
-                       c_data, kept_out, tid, kLoop ? slot_counter : nullptr, &next_slot);
+                       c_data, kept_out, tid,
+                       kLoop ? slot_counter + (blockIdx.x & 7) : nullptr, &next_slot);
     if constexpr (!kLoop) break;
     __syncthreads();  // next_slot is visible; the next set-up rewrites the LDS
-    slot = __builtin_amdgcn_readfirstlane(next_slot);
+    slot = (int32_t)(blockIdx.x & 7) + 8 * __builtin_amdgcn_readfirstlane(next_slot);
     if (slot >= n_tiles) break;
   }
 }
@@ -410,10 +413,12 @@ __global__ __launch_bounds__(256, 2) void numeric_f64_ksplit_kernel(
     const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
     double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
   LAMB_GUARD;
-  // slots: blockIdx.x first; with kLoop (a persistent grid) the next ones
-  // from the counter (set to the grid size before the launch), claimed
-  // during each tile's epilogue. Without kLoop the body runs once, as a
-  // plain one-tile kernel with that kernel's registers.
+  // slots: blockIdx.x first. With kLoop (a persistent grid) a workgroup then
+  // takes slots x, x + 8, x + 16, ... of its residue x = blockIdx.x mod 8 (the
+  // XCD the hardware deals it to) from counter x, one claimed during each
+  // tile's epilogue: each XCD walks the same tiles, in the same order, as
+  // with one workgroup per tile. Without kLoop the body runs once, as a plain
+  // one-tile kernel with that kernel's registers.
   __shared__ int32_t next_slot;
   for (int32_t slot = blockIdx.x;;) {
     __shared__ double lds[kBandRows * kPitch];
@@ -553,10 +558,11 @@ __global__ __launch_bounds__(256, 2) void numeric_f64_ksplit_kernel(
       __syncthreads();
     }
     tile_epilogue<256>(Ct, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data,
-                       c_data, kept_out, tid, kLoop ? slot_counter : nullptr, &next_slot);
+                       c_data, kept_out, tid,
+                       kLoop ? slot_counter + (blockIdx.x & 7) : nullptr, &next_slot);
     if constexpr (!kLoop) break;
     __syncthreads();  // next_slot is visible; the next set-up rewrites the LDS
-    slot = __builtin_amdgcn_readfirstlane(next_slot);
+    slot = (int32_t)(blockIdx.x & 7) + 8 * __builtin_amdgcn_readfirstlane(next_slot);
     if (slot >= n_tiles) break;
   }
 }
@@ -586,10 +592,12 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
     const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
     double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
   LAMB_GUARD;
-  // slots: blockIdx.x first; with kLoop (a persistent grid) the next ones
-  // from the counter (set to the grid size before the launch), claimed
-  // during each tile's epilogue. Without kLoop the body runs once, as a
-  // plain one-tile kernel with that kernel's registers.
+  // slots: blockIdx.x first. With kLoop (a persistent grid) a workgroup then
+  // takes slots x, x + 8, x + 16, ... of its residue x = blockIdx.x mod 8 (the
+  // XCD the hardware deals it to) from counter x, one claimed during each
+  // tile's epilogue: each XCD walks the same tiles, in the same order, as
+  // with one workgroup per tile. Without kLoop the body runs once, as a plain
+  // one-tile kernel with that kernel's registers.
   __shared__ int32_t next_slot;
   for (int32_t slot = blockIdx.x;;) {
     __shared__ double lds[kBandRows * kPitch];
@@ -735,10 +743,11 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
       __syncthreads();
     }
     tile_epilogue<256>(Ct, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data,
-                       c_data, kept_out, tid, kLoop ? slot_counter : nullptr, &next_slot);
+                       c_data, kept_out, tid,
+                       kLoop ? slot_counter + (blockIdx.x & 7) : nullptr, &next_slot);
     if constexpr (!kLoop) break;
     __syncthreads();  // next_slot is visible; the next set-up rewrites the LDS
-    slot = __builtin_amdgcn_readfirstlane(next_slot);
+    slot = (int32_t)(blockIdx.x & 7) + 8 * __builtin_amdgcn_readfirstlane(next_slot);
     if (slot >= n_tiles) break;
   }
 }
@@ -765,16 +774,18 @@ int32_t resident_workgroups(const void* kernel) {
   return n;
 }
 
-// Sets a persistent grid's slot counter to its first dynamic slot (the grid
-// size: workgroup b takes slot b first). A kernel, not a memset, so that
-// graph captures replay it reliably.
-__global__ void slot_counter_reset(int32_t* __restrict__ counter, int32_t first) {
+// Sets a persistent grid's eight slot counters: counter x counts the slots
+// x + 8k already taken by the workgroups of residue x (blockIdx.x mod 8),
+// which take slots x, x + 8, ... below the grid size first. A kernel, not a
+// memset, so that graph captures replay it.
+__global__ void slot_counter_reset(int32_t* __restrict__ counter, int32_t grid) {
   LAMB_GUARD;
-  *counter = first;
+  const int x = threadIdx.x;
+  if (x < 8) counter[x] = (grid - x + 7) / 8;
 }
 
-void launch_slot_counter_reset(hipStream_t s, int32_t* counter, int32_t first) {
-  hipLaunchKernelGGL(slot_counter_reset, dim3(1), dim3(1), 0, s, counter, first);
+void launch_slot_counter_reset(hipStream_t s, int32_t* counter, int32_t grid) {
+  hipLaunchKernelGGL(slot_counter_reset, dim3(1), dim3(64), 0, s, counter, grid);
 }
 
 hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* tiles,

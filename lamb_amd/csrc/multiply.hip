@@ -335,7 +335,7 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
   C->s_data = DevBuf();
   HIPCHK(ctx, newdata.ensure(sizeof(double) * (size_t)std::max<int64_t>(S.r.c_size, 1)),
          "C pool");
-  HIPCHK(ctx, ctx->d_slot_counter.ensure(sizeof(int32_t)), "slot counter");
+  HIPCHK(ctx, ctx->d_slot_counter.ensure(8 * sizeof(int32_t)), "slot counters");
   int32_t grid = 0;
   int32_t* slot_counter = (int32_t*)ctx->d_slot_counter.p;
   auto launch_numeric = [&](bool persistent) -> hipError_t {

@@ -193,8 +193,9 @@ __device__ __forceinline__ uint32_t quadrant_blocks(uint32_t g, uint32_t gsh_r, 
 // column-major order, s = fma(x, x, s) from s = +0, the same order as the
 // prefilter's block norms; the block is kept iff that sum is >= eps2.
 // kept_out[tile*16 + i] gets the kept columns of band atom i. With a
-// slot_counter (a persistent grid), thread 0 takes the workgroup's next tile
-// slot from it and stores it to *next_slot (LDS), visible after a barrier.
+// slot_counter (a persistent grid), thread 0 takes the next value of that
+// counter (the index of the workgroup's next slot in its sequence) and
+// stores it to *next_slot (LDS), visible after a barrier.
 template <int NT>
 __device__ __forceinline__ void tile_epilogue(
     double* Ct, const TileInfo& t, TileSmem& sm, double alpha, double beta, double eps2,
