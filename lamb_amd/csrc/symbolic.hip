@@ -665,6 +665,15 @@ __global__ __launch_bounds__(64) void cidx_count(int32_t nbands, const Band* __r
   if (lane < nat) row_cnt[bands[I].first + lane] = tot;
 }
 
+// Writes the C index of (row band I, segment g): for each kept block of the
+// segment's tiles its column atom, C pool offset and row atom, at its CSR
+// position (a band's tiles are in J order and a tile's columns ascend, so
+// each row's blocks come in column order). Passes of 16 tiles. First, lanes
+// over the 16 rows of 4 tiles at a time: each (tile, row)'s offset inside
+// its tile's C region (laid out row by row), pattern and kept masks, and the
+// tile's column band, into LDS. Then lanes over the 16 tiles of 4 rows at a
+// time write the index: the lanes of a row write neighbouring CSR positions,
+// where lanes over rows would each write a different part of the arrays.
 __global__ __launch_bounds__(64) void cidx_fill(int32_t nbands, const Band* __restrict__ bands,
                                                 const int64_t* __restrict__ tile_base,
                                                 const TileDesc* __restrict__ tiles,
@@ -679,60 +688,102 @@ __global__ __launch_bounds__(64) void cidx_fill(int32_t nbands, const Band* __re
   const int g = (int)(blockIdx.x % kCidxSeg);
   if (I >= nbands) return;
   const int lane = threadIdx.x;
+  constexpr int kPass = 16;  // tiles per pass
+  __shared__ int64_t s_next[kBandAtoms];               // next CSR position of row p
+  __shared__ int32_t s_m[kBandAtoms];                  // rows of band I's atom p
+  __shared__ int64_t s_off[kPass][kBandAtoms];         // C pool offset of row p in tile l
+  __shared__ uint16_t s_pm[kPass][kBandAtoms], s_km[kPass][kBandAtoms];
+  __shared__ int16_t s_rs[kPass][kBandAtoms + 1];      // column starts of tile l's band J
+  __shared__ int32_t s_jfirst[kPass], s_jnat[kPass];
   const Band& bI = bands[I];
-  // four tiles per pass: lane group `slot` (16 lanes) takes tile tb + slot,
-  // lane p of the group row atom p of band I
-  const int p = lane & 15, slot = lane >> 4;
-  const bool act = p < bI.natoms;
-  const int m = act ? bI.rs[p + 1] - bI.rs[p] : 0;
-  // next C block of row p (same in the 4 groups): after the earlier segments
-  int64_t b = 0;
-  if (act) {
-    b = row_p[bI.first + p];
-    for (int h = 0; h < g; ++h) b += seg_cnt[((int64_t)I * kCidxSeg + h) * 16 + p];
+  const int nat = bI.natoms;
+  if (lane < kBandAtoms) {
+    int64_t b = 0;
+    int m = 0;
+    if (lane < nat) {
+      b = row_p[bI.first + lane];
+      for (int h = 0; h < g; ++h) b += seg_cnt[((int64_t)I * kCidxSeg + h) * 16 + lane];
+      m = bI.rs[lane + 1] - bI.rs[lane];
+    }
+    s_next[lane] = b;
+    s_m[lane] = m;
   }
+  __syncthreads();
   unsigned long long ne = 0;
   int64_t t0, t1;
   cidx_segment(tile_base, I, g, t0, t1);
-  for (int64_t tb = t0; tb < t1; tb += 4) {
-    const int64_t t = tb + slot;
-    const bool tv = t < t1;
-    const TileDesc* td = tiles + (tv ? t : tb);
-    const Band& bJ = bands[td->J];
-    const uint32_t pm = (act && tv) ? td->pattern[p] : 0u;
-    const uint32_t km = (act && tv) ? kept[t * kBandAtoms + p] : 0u;
-    int64_t rowsz = 0;
-    for (int q = 0; q < bJ.natoms; ++q)
-      if ((pm >> q) & 1u) rowsz += (int64_t)m * (bJ.rs[q + 1] - bJ.rs[q]);
-    // exclusive prefix over the 16 row positions of this tile
-    int64_t x = rowsz;
-    for (int o = 1; o < 16; o <<= 1) {
-      const int64_t y = __shfl_up(x, o, 16);
-      if (p >= o) x += y;
-    }
-    int64_t o_run = td->out_base + x - rowsz;
-    // kept blocks of row p in the earlier tiles of this pass
-    const int nk = __builtin_popcount(pm & km);
-    int kx = nk;
-    for (int o = 16; o < 64; o <<= 1) {
-      const int y = __shfl_up(kx, o, 64);
-      if (lane >= o) kx += y;
-    }
-    const int total = __shfl(kx, 48 + p, 64);
-    int64_t bb = b + (kx - nk);
-    for (int q = 0; q < bJ.natoms; ++q) {
-      if (!((pm >> q) & 1u)) continue;
-      const int64_t sz = (int64_t)m * (bJ.rs[q + 1] - bJ.rs[q]);
-      if ((km >> q) & 1u) {
-        col[bb] = bJ.first + q;
-        off[bb] = o_run;
-        rows[bb] = bI.first + p;
-        ++bb;
-        ne += (unsigned long long)sz;
+  for (int64_t tb = t0; tb < t1; tb += kPass) {
+    // ---- row offsets, masks and column bands of the pass's tiles ----
+    {
+      const int p = lane & 15, slot = lane >> 4;
+      for (int sp = 0; sp < kPass / 4; ++sp) {
+        const int l = 4 * sp + slot;
+        const int64_t t = tb + l;
+        const bool tv = t < t1;
+        const TileDesc* td = tiles + (tv ? t : tb);
+        const Band& bJ = bands[td->J];
+        const bool act = tv && p < nat;
+        const uint32_t pm = act ? td->pattern[p] : 0u;
+        const int m = s_m[p];
+        const int jn = bJ.natoms;
+        int64_t rowsz = 0;
+        for (int q = 0; q < jn; ++q)
+          if ((pm >> q) & 1u) rowsz += (int64_t)m * (bJ.rs[q + 1] - bJ.rs[q]);
+        // exclusive prefix over the 16 rows of this tile
+        int64_t x = rowsz;
+        for (int o = 1; o < 16; o <<= 1) {
+          const int64_t y = __shfl_up(x, o, 16);
+          if (p >= o) x += y;
+        }
+        s_off[l][p] = td->out_base + x - rowsz;
+        s_pm[l][p] = (uint16_t)pm;
+        s_km[l][p] = act ? kept[t * kBandAtoms + p] : (uint16_t)0;
+        s_rs[l][p] = bJ.rs[min(p, jn)];
+        if (p == 0) {
+          s_rs[l][kBandAtoms] = bJ.rs[jn];
+          s_jfirst[l] = bJ.first;
+          s_jnat[l] = tv ? jn : 0;
+        }
       }
-      o_run += sz;
     }
-    b += total;
+    __syncthreads();
+    // ---- the index: lane (r, l) = row 4 * round + r of tile tb + l ----
+    {
+      const int l = lane & 15, r = lane >> 4;
+      const int jn = s_jnat[l];
+      for (int round = 0; round < kBandAtoms / 4; ++round) {
+        const int p = 4 * round + r;
+        if (4 * round >= nat) break;  // no row of this round in band I
+        const uint32_t pm = p < nat ? s_pm[l][p] : 0u;
+        const uint32_t km = s_km[l][p];
+        const int nk = __builtin_popcount(pm & km);
+        int x = nk;  // inclusive prefix over the 16 tiles of this row
+        for (int o = 1; o < 16; o <<= 1) {
+          const int y = __shfl_up(x, o, 16);
+          if (l >= o) x += y;
+        }
+        const int total = __shfl(x, 15, 16);
+        const int64_t first = s_next[p];
+        int64_t bb = first + x - nk;
+        int64_t o_run = s_off[l][p];
+        const int m = s_m[p];
+        const int32_t jf = s_jfirst[l];
+        for (int q = 0; q < jn; ++q) {
+          if (!((pm >> q) & 1u)) continue;
+          const int64_t sz = (int64_t)m * (s_rs[l][q + 1] - s_rs[l][q]);
+          if ((km >> q) & 1u) {
+            col[bb] = jf + q;
+            off[bb] = o_run;
+            rows[bb] = bI.first + p;
+            ++bb;
+            ne += (unsigned long long)sz;
+          }
+          o_run += sz;
+        }
+        if (l == 15 && p < nat) s_next[p] = first + total;
+      }
+    }
+    __syncthreads();  // the next pass rewrites the LDS
   }
   ne = wave_sum_i64((int64_t)ne);
   if (lane == 0) atomicAdd(nelems, ne);
