@@ -67,6 +67,17 @@ struct Strip {
   double nsq[kBandAtoms];
 };
 
+// The fields of a strip that the symbolic kernels read for every strip pair
+// (16 bytes in place of the 280 of a Strip): its band, atom mask, the rows
+// (A) or columns (B) of the band the mask covers, and its smallest block norm.
+struct StripHead {
+  int32_t band;
+  uint16_t mask;
+  uint16_t rows;
+  double min_nsq;
+};
+static_assert(sizeof(StripHead) == 16, "StripHead layout");
+
 struct Band {
   int32_t first;  // first internal atom
   int32_t natoms;

@@ -35,10 +35,12 @@ struct StripSet {
   int64_t n = 0;         // strips
   int64_t pk_total = 0;  // packed panel elements
   DevBuf strips;         // Strip[n]
+  DevBuf head;           // StripHead[n]
   DevBuf ptr;            // int64: per band (A) or per row atom (B), +1
   DevBuf pk;             // int64[n+1]: packed panel offsets
   void release() {
     strips.release();
+    head.release();
     ptr.release();
     pk.release();
   }

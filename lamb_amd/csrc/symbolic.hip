@@ -130,12 +130,25 @@ __global__ void strip_fill(int64_t nblk, int a_side, const int64_t* __restrict__
 }
 
 // per-strip panel size: K columns rounded up to a multiple of 8 (one BF16
-// slice chunk), times 64
+// slice chunk), times 64; and the strip's head
 __global__ void strip_panel_size(int64_t n, const Strip* __restrict__ s,
-                                 const int32_t* __restrict__ blk, int64_t* __restrict__ sz) {
+                                 const int32_t* __restrict__ blk, const Band* __restrict__ bands,
+                                 int64_t* __restrict__ sz, StripHead* __restrict__ head) {
   LAMB_GUARD;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) sz[i] = 64 * (int64_t)((blk[s[i].k] + 7) & ~7);
+  if (i >= n) return;
+  sz[i] = 64 * (int64_t)((blk[s[i].k] + 7) & ~7);
+  const int32_t band = s[i].band;
+  const uint32_t m = s[i].mask;
+  int rows = 0;
+  for (int p = 0; p < kBandAtoms; ++p)
+    if ((m >> p) & 1u) rows += bands[band].rs[p + 1] - bands[band].rs[p];
+  StripHead h;
+  h.band = band;
+  h.mask = (uint16_t)m;
+  h.rows = (uint16_t)rows;
+  h.min_nsq = s[i].min_nsq;
+  head[i] = h;
 }
 
 // ptr[g] = first strip of group g: strip index at the group's first slot
@@ -209,11 +222,14 @@ struct StripWalk {
 
 // Calls emit(rm, cm) for every entry of strip pair (SA, SB) under the
 // prefilter nsq(A_ik) * nsq(B_kj) >= eps2; returns the dropped pair count.
+// The heads decide the common case (no pair cut); the strips' per-atom norms
+// are read only when the smallest norms could be cut.
 template <typename F>
-__device__ __forceinline__ int pair_entries(const Strip* SA, const Strip* SB, double eps2,
+__device__ __forceinline__ int pair_entries(const StripHead& HA, const StripHead& HB,
+                                            const Strip* SA, const Strip* SB, double eps2,
                                             F&& emit) {
-  const uint32_t am = SA->mask, bm = SB->mask;
-  if (SA->min_nsq * SB->min_nsq >= eps2) {
+  const uint32_t am = HA.mask, bm = HB.mask;
+  if (HA.min_nsq * HB.min_nsq >= eps2) {
     emit(am, bm);
     return 0;
   }
@@ -297,8 +313,9 @@ __global__ void mask_pattern(int64_t nblk, const int32_t* __restrict__ rows,
 template <bool kLds>
 __global__ __launch_bounds__(kLds ? 512 : 64) void sym_count(
     int32_t nbands, int nseg, const Band* __restrict__ bands, const int32_t* __restrict__ blk,
-    const Strip* __restrict__ as, const int64_t* __restrict__ a_ptr,
-    const Strip* __restrict__ bs, const int64_t* __restrict__ b_ptr, double eps2,
+    const Strip* __restrict__ as, const StripHead* __restrict__ ah,
+    const int64_t* __restrict__ a_ptr, const Strip* __restrict__ bs,
+    const StripHead* __restrict__ bh, const int64_t* __restrict__ b_ptr, double eps2,
     const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
     const int32_t* __restrict__ band_of, const int32_t* __restrict__ pos_in_band,
     int32_t* __restrict__ cnt, int32_t* __restrict__ ksum, uint32_t* __restrict__ pat,
@@ -335,28 +352,33 @@ __global__ __launch_bounds__(kLds ? 512 : 64) void sym_count(
   StripWalk walk(as, blk, b_ptr, lo, hi);
   for (int64_t sa = lo; sa < hi; ++sa) {
     const Strip* SA = as + sa;
+    const StripHead HA = ah[sa];
     const int32_t k = walk.k1;
     const int kd = walk.kd1;
     const int64_t b0 = walk.b01, b1 = walk.b11;
     walk.advance(sa);
     for (int64_t sb = b0 + lane; sb < b1; sb += 64) {
-      const Strip* SB = bs + sb;
-      const int32_t J = SB->band;
+      const StripHead HB = bh[sb];
+      const int32_t J = HB.band;
       int n = 0;
-      dropped += pair_entries(SA, SB, eps2, [&](uint32_t rm, uint32_t cm) {
+      dropped += pair_entries(HA, HB, SA, bs + sb, eps2, [&](uint32_t rm, uint32_t cm) {
         uint32_t pairs[kBandAtoms];
         if (!mask_entry(mpat, nbands, I, J, rm, cm, pairs)) return;
         // a zero-size K atom adds to the C pattern but holds no entry
         if (kd > 0) ++n;
         // entries whose atoms the prefilter or the mask cut (sym_fill flags
         // them with kEntryPartial)
-        if (kd > 0 && (rm != SA->mask || cm != SB->mask)) ++partial;
+        if (kd > 0 && (rm != HA.mask || cm != HB.mask)) ++partial;
         kept += (unsigned long long)__builtin_popcount(rm) * __builtin_popcount(cm);
-        // rows of cm in band J
-        int cr = 0;
-        for (int q = 0; q < kBandAtoms; ++q)
-          if ((cm >> q) & 1u) cr += bands[J].rs[q + 1] - bands[J].rs[q];
-        flops += 2.0 * rows_of_mask(rsI, rm) * kd * cr;
+        // rows of rm in band I, columns of cm in band J
+        int cr = HB.rows;
+        if (cm != HB.mask) {
+          cr = 0;
+          for (int q = 0; q < kBandAtoms; ++q)
+            if ((cm >> q) & 1u) cr += bands[J].rs[q + 1] - bands[J].rs[q];
+        }
+        const int rr = rm == HA.mask ? (int)HA.rows : rows_of_mask(rsI, rm);
+        flops += 2.0 * rr * kd * cr;
         // one atomic per pattern word (two row atoms)
         for (int wd = 0; wd < kBandAtoms / 2; ++wd) {
           const uint32_t v = pairs[2 * wd] | (pairs[2 * wd + 1] << 16);
@@ -559,7 +581,8 @@ __global__ __launch_bounds__(64) void sym_tiles(
 template <bool kLds>
 __global__ __launch_bounds__(kLds ? 512 : 64) void sym_fill(
     int32_t nbands, int nseg, const int32_t* __restrict__ blk, const Strip* __restrict__ as,
-    const int64_t* __restrict__ a_ptr, const Strip* __restrict__ bs,
+    const StripHead* __restrict__ ah, const int64_t* __restrict__ a_ptr,
+    const Strip* __restrict__ bs, const StripHead* __restrict__ bh,
     const int64_t* __restrict__ b_ptr, const int64_t* __restrict__ a_pk,
     const int64_t* __restrict__ b_pk, double eps2, int32_t* __restrict__ cursor,
     int32_t* __restrict__ kcur, Entry* __restrict__ entries, const uint32_t* __restrict__ mpat) {
@@ -585,6 +608,7 @@ __global__ __launch_bounds__(kLds ? 512 : 64) void sym_fill(
   StripWalk walk(as, blk, b_ptr, lo, hi);
   for (int64_t sa = lo; sa < hi; ++sa) {
     const Strip* SA = as + sa;
+    const StripHead HA = ah[sa];
     const int32_t k = walk.k1;
     const int kd = walk.kd1;
     const int64_t b0 = walk.b01, b1 = walk.b11;
@@ -592,10 +616,11 @@ __global__ __launch_bounds__(kLds ? 512 : 64) void sym_fill(
     (void)k;
     for (int64_t sb = b0 + lane; sb < b1; sb += 64) {
       const Strip* SB = bs + sb;
-      const int32_t J = SB->band;
+      const StripHead HB = bh[sb];
+      const int32_t J = HB.band;
       if (kd <= 0) continue;
       int n = 0;
-      pair_entries(SA, SB, eps2, [&](uint32_t rm, uint32_t cm) {
+      pair_entries(HA, HB, SA, SB, eps2, [&](uint32_t rm, uint32_t cm) {
         uint32_t pairs[kBandAtoms];
         if (mask_entry(mpat, nbands, I, J, rm, cm, pairs)) ++n;
       });
@@ -604,12 +629,12 @@ __global__ __launch_bounds__(kLds ? 512 : 64) void sym_fill(
       // of one step hold distinct J and steps run in K order
       int32_t e = atomicAdd(&c[J], n);
       int32_t kk = atomicAdd(&kc[J], n * kd);
-      pair_entries(SA, SB, eps2, [&](uint32_t rm, uint32_t cm) {
+      pair_entries(HA, HB, SA, SB, eps2, [&](uint32_t rm, uint32_t cm) {
         uint32_t pairs[kBandAtoms];
         if (!mask_entry(mpat, nbands, I, J, rm, cm, pairs)) return;
         Entry en;
         en.a_col = (uint32_t)(a_pk[sa] / 64) |
-                   ((rm == SA->mask && cm == SB->mask) ? 0u : kEntryPartial);
+                   ((rm == HA.mask && cm == HB.mask) ? 0u : kEntryPartial);
         en.b_col = (uint32_t)(b_pk[sb] / 64);
         en.kstart = kk;
         en.rmask = (uint16_t)rm;
@@ -899,9 +924,11 @@ hipError_t build_strips(lamb_hip_ctx* ctx, SymScratch& W, const MatView& M, cons
   CK(out.pk.ensure(sizeof(int64_t) * (size_t)(nstrips + 1)));
   CK(W.flag.ensure(sizeof(int64_t) * (size_t)(std::max(nblk, nstrips) + 1)));
   CK(W.scan.ensure(sizeof(int64_t) * (size_t)scan_scratch_elems(std::max(nblk, nstrips) + 1)));
+  CK(out.head.ensure(sizeof(StripHead) * (size_t)std::max<int64_t>(nstrips, 1)));
   if (nstrips)
     hipLaunchKernelGGL(strip_panel_size, dim3(grid_of(nstrips)), dim3(256), 0, s, nstrips,
-                       (const Strip*)out.strips.p, D.blk, (int64_t*)W.flag.p);
+                       (const Strip*)out.strips.p, D.blk, D.bands, (int64_t*)W.flag.p,
+                       (StripHead*)out.head.p);
   CK(exclusive_scan(s, (const int64_t*)W.flag.p, (int64_t*)out.pk.p, nstrips, (int64_t*)W.scan.p));
   CK(read_sizes(ctx, (int64_t*)out.pk.p + nstrips, &out.pk_total, 1));
   return hipSuccess;
@@ -942,15 +969,15 @@ hipError_t build_tiles(lamb_hip_ctx* ctx, SymScratch& W, const DistView& D, cons
   if (nb) {
     if (cnt_lds)
       hipLaunchKernelGGL(sym_count<true>, dim3((unsigned)nb), dim3(64 * nseg), lds_cnt, s, nb, nseg, D.bands,
-                       D.blk, (const Strip*)A.strips.p, (const int64_t*)A.ptr.p,
-                       (const Strip*)B.strips.p, (const int64_t*)B.ptr.p, eps2,
+                       D.blk, (const Strip*)A.strips.p, (const StripHead*)A.head.p, (const int64_t*)A.ptr.p,
+                       (const Strip*)B.strips.p, (const StripHead*)B.head.p, (const int64_t*)B.ptr.p, eps2,
                        Cold ? Cold->row_p : nullptr, Cold ? Cold->col : nullptr, D.band_of,
                        D.pos_in_band, (int32_t*)W.cnt.p, (int32_t*)W.ksum.p,
                        (uint32_t*)W.pat.p, (unsigned long long*)W.stats.p, mp);
     else
       hipLaunchKernelGGL(sym_count<false>, dim3((unsigned)(nb * nseg)), dim3(64), 0, s, nb, nseg, D.bands,
-                       D.blk, (const Strip*)A.strips.p, (const int64_t*)A.ptr.p,
-                       (const Strip*)B.strips.p, (const int64_t*)B.ptr.p, eps2,
+                       D.blk, (const Strip*)A.strips.p, (const StripHead*)A.head.p, (const int64_t*)A.ptr.p,
+                       (const Strip*)B.strips.p, (const StripHead*)B.head.p, (const int64_t*)B.ptr.p, eps2,
                        Cold ? Cold->row_p : nullptr, Cold ? Cold->col : nullptr, D.band_of,
                        D.pos_in_band, (int32_t*)W.cnt.p, (int32_t*)W.ksum.p,
                        (uint32_t*)W.pat.p, (unsigned long long*)W.stats.p, mp);
@@ -1001,14 +1028,14 @@ hipError_t build_tiles(lamb_hip_ctx* ctx, SymScratch& W, const DistView& D, cons
     const size_t lds_fill = sym_fill_lds(nb, nseg);
     if (lds_fill <= kSymLdsMax && sym_lds_allowed())
       hipLaunchKernelGGL(sym_fill<true>, dim3((unsigned)nb), dim3(64 * nseg), lds_fill, s, nb,
-                         nseg, D.blk, (const Strip*)A.strips.p, (const int64_t*)A.ptr.p,
-                         (const Strip*)B.strips.p, (const int64_t*)B.ptr.p,
+                         nseg, D.blk, (const Strip*)A.strips.p, (const StripHead*)A.head.p, (const int64_t*)A.ptr.p,
+                         (const Strip*)B.strips.p, (const StripHead*)B.head.p, (const int64_t*)B.ptr.p,
                          (const int64_t*)A.pk.p, (const int64_t*)B.pk.p, eps2,
                          (int32_t*)W.cnt.p, (int32_t*)W.ksum.p, (Entry*)r.entries.p, mp);
     else
       hipLaunchKernelGGL(sym_fill<false>, dim3((unsigned)(nb * nseg)), dim3(64), 0, s, nb, nseg,
-                         D.blk, (const Strip*)A.strips.p, (const int64_t*)A.ptr.p,
-                         (const Strip*)B.strips.p, (const int64_t*)B.ptr.p,
+                         D.blk, (const Strip*)A.strips.p, (const StripHead*)A.head.p, (const int64_t*)A.ptr.p,
+                         (const Strip*)B.strips.p, (const StripHead*)B.head.p, (const int64_t*)B.ptr.p,
                          (const int64_t*)A.pk.p, (const int64_t*)B.pk.p, eps2,
                          (int32_t*)W.cnt.p, (int32_t*)W.ksum.p, (Entry*)r.entries.p, mp);
   }
