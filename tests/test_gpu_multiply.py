@@ -360,3 +360,45 @@ def test_symbolic_global_memory_forms(ctx, monkeypatch, case):
     monkeypatch.setenv("LAMB_HIP_SYM_LDS", "0")
     {"water_eps": test_water_eps, "filters_bind": test_filters_bind,
      "alpha_beta": test_alpha_beta, "three_kinds": test_three_kinds}[case](ctx)
+
+
+@pytest.mark.parametrize("size", [5, 23, 64])
+def test_block_norm_order_near_threshold(ctx, size):
+    """Block norms (the prefilter's and the stand-alone post-filter's) are one
+    sequential column-major FMA chain per block, including blocks longer than
+    one LDS chunk of the norm kernel (23 x 23 = 529 and 64 x 64 = 4,096
+    values). eps is placed so that fl(eps * eps) lies between the chain's sum
+    and the reversed chain's for one block: the filter must keep or drop it as
+    the oracle's sequential norm says."""
+    atoms, x = S.small_system(24 if size == 64 else 40, 17 + size, sizes=(size,),
+                              cutoff=7.0, decay=0.3)
+    seq = oracle.block_nsq(x)
+    pick = None
+    for t in range(x.n_blocks):
+        v = x.data[x.blk_off[t]:x.blk_off[t] + size * size]
+        alt = oracle.nsq1(v[::-1].copy())
+        if alt != seq[t]:
+            lo, hi = sorted((alt, float(seq[t])))
+            eps = _eps_between(lo, hi)
+            if eps is not None:
+                pick = (t, eps)
+                break
+    assert pick is not None, "no block whose two summation orders differ"
+    t, eps = pick
+    keep = seq >= eps * eps
+    print(f"case: {size}x{size} blocks, block {t} seq={seq[t]!r} eps^2={eps * eps!r} "
+          f"kept {int(keep.sum())} of {x.n_blocks}", flush=True)
+    dist = Distribution(atoms.blk, atoms.pos, atoms.box)
+    dd = D.DeviceDistribution(ctx, dist)
+    dm = D.DeviceMatrix.from_lamb(dd, LambMatrix.from_global(dist, x))
+    try:
+        dropped = D.matrix_filter(dm, eps)
+        got = dm.download().to_global()
+    finally:
+        dm.close()
+        dd.close()
+    assert dropped == int((~keep).sum())
+    rows = np.repeat(np.arange(x.n_atoms), np.diff(x.row_p))
+    want = set(zip(rows[keep].tolist(), x.col[keep].tolist()))
+    grows = np.repeat(np.arange(got.n_atoms), np.diff(got.row_p))
+    assert set(zip(grows.tolist(), got.col.tolist())) == want
