@@ -26,9 +26,12 @@ PEAK_HBM_GBS = 8000.0
 # profiles/numeric_f64_half_kernel_water6144.md. A kernel, order or launch
 # form without a profile reports null.
 PROFILED_TRAFFIC_BYTES = {
-    ("numeric_f64_half_kernel", 4, False): 104.85e9,
-    ("numeric_f64_half_kernel", 8, False): 104.04e9,
-    ("numeric_f64_half_kernel", 16, False): 117.92e9,
+    ("numeric_f64_half_kernel", 4, False): 109.94e9,
+    ("numeric_f64_half_kernel", 4, True): 117.64e9,
+    ("numeric_f64_half_kernel", 8, False): 108.31e9,
+    ("numeric_f64_half_kernel", 8, True): 114.90e9,
+    ("numeric_f64_half_kernel", 16, False): 121.87e9,
+    ("numeric_f64_half_kernel", 16, True): 128.66e9,
 }
 # FP64 numeric kernel of each variant (lamb_hip_tuning_info)
 F64_KERNELS = {0: "numeric_f64_kernel", 1: "numeric_f64_ksplit_kernel",
