@@ -20,19 +20,23 @@ METRIC = "block-SpGEMM GFLOP/s & HBM GB/s: FP64 plain vs BF16x3 batched at ≤1e
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of an FP64 numeric kernel on this workload, by kernel
-# name, tile-order row block and launch form (persistent grid or not):
+# name, tile-order row block and launch form (0 one workgroup per tile, 1
+# persistent grid over the interleaved order, 2 persistent grid with one range
+# of the order per XCD):
 # rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scale factors from a
 # device copy of known size in the same run), from
 # profiles/numeric_f64_half_kernel_water6144.md. A kernel, order or launch
 # form without a profile reports null.
 PROFILED_TRAFFIC_BYTES = {
-    ("numeric_f64_half_kernel", 4, False): 109.94e9,
-    ("numeric_f64_half_kernel", 4, True): 117.64e9,
-    ("numeric_f64_half_kernel", 8, False): 108.31e9,
-    ("numeric_f64_half_kernel", 8, True): 114.90e9,
-    ("numeric_f64_half_kernel", 16, False): 121.87e9,
-    ("numeric_f64_half_kernel", 16, True): 128.66e9,
+    ("numeric_f64_half_kernel", 4, 0): 109.94e9,
+    ("numeric_f64_half_kernel", 4, 1): 117.64e9,
+    ("numeric_f64_half_kernel", 8, 0): 108.31e9,
+    ("numeric_f64_half_kernel", 8, 1): 114.90e9,
+    ("numeric_f64_half_kernel", 16, 0): 121.87e9,
+    ("numeric_f64_half_kernel", 16, 1): 128.66e9,
 }
+LAUNCH_FORMS = {0: "one workgroup per tile", 1: "persistent grid, XCDs share one window",
+                2: "persistent grid, one range of the tile order per XCD"}
 # FP64 numeric kernel of each variant (lamb_hip_tuning_info)
 F64_KERNELS = {0: "numeric_f64_kernel", 1: "numeric_f64_ksplit_kernel",
                2: "numeric_f64_half_kernel"}
@@ -583,7 +587,7 @@ def main():
     variant, _ = D.tuning_info(ctx)
     kname = F64_KERNELS.get(variant, str(variant))
     rowblock = D.tuning_order(ctx)
-    persistent, workgroups, _ = D.tuning_launch(ctx)
+    form, workgroups, _ = D.tuning_launch(ctx)
     result = {
         "metric": METRIC,
         "value": flops / (ms_step * 1e-3) / 1e9,
@@ -627,9 +631,9 @@ def main():
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tf / PEAK_FP64_TFLOPS,
-            "traffic": PROFILED_TRAFFIC_BYTES.get((kname, rowblock, persistent)),
+            "traffic": PROFILED_TRAFFIC_BYTES.get((kname, rowblock, form)),
             "tile_order_rowblock": rowblock,
-            "persistent_grid": persistent,
+            "launch_form": LAUNCH_FORMS.get(form, str(form)),
             "workgroups": workgroups,
             "algorithmic_bytes": bytes_alg,
         },

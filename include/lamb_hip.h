@@ -218,12 +218,14 @@ int32_t lamb_hip_trace_noalloc(const lamb_hip_mat* A, double* dev_trace);
  * quadrant|half|ksplit forces one (the variants add K products in different
  * orders: results then differ at rounding level). n_classes: workload classes
  * tuned so far. The autotuner times the tile orders (row bands per block of
- * the order in which workgroups take tiles: 4, 8, 16), each launched with one
- * workgroup per tile and as a persistent grid (the device's resident
- * workgroups, each looping over tiles), on the first product of a class and
- * keeps the fastest; order and launch form only map tiles to workgroups, so
- * every candidate writes the same bits. LAMB_HIP_TILE_ROWBLOCK=n and
- * LAMB_HIP_PERSISTENT=0|1 force them. */
+ * the order in which workgroups take tiles: 4, 8, 16), each in three launch
+ * forms (one workgroup per tile; a persistent grid of the device's resident
+ * workgroups, each looping over tiles, the XCDs sharing one window of the
+ * order; a persistent grid whose XCDs each walk their own work-balanced range
+ * of the order), on the first product of a class and keeps the fastest; order
+ * and launch form only map tiles to workgroups, so every candidate writes the
+ * same bits. LAMB_HIP_TILE_ROWBLOCK=n and LAMB_HIP_PERSISTENT=0|1|2 force
+ * them. */
 int32_t lamb_hip_tuning_info(const lamb_hip_ctx* ctx, int32_t* last_variant,
                              int32_t* n_classes);
 /* Row bands per block of the tile order the context's last multiply used (the
@@ -251,9 +253,12 @@ int32_t lamb_hip_tuning_mode(lamb_hip_ctx* ctx, int32_t tune_shapes);
 int32_t lamb_hip_tuning_save(const lamb_hip_ctx* ctx, const char* path);
 int32_t lamb_hip_tuning_load(lamb_hip_ctx* ctx, const char* path);
 /* Launch form of the context's last numeric kernel (no counterpart in the
- * reference): persistent = 1 when it ran as a persistent grid, workgroups =
- * the grid launched, tiles = the product's C tiles (-1 before any multiply). */
-int32_t lamb_hip_tuning_launch(const lamb_hip_ctx* ctx, int32_t* persistent,
+ * reference): form = 0 for one workgroup per tile, 1 for a persistent grid
+ * whose XCDs share one window of the tile order, 2 for a persistent grid
+ * whose XCDs each walk their own range of it (work-balanced by K columns);
+ * workgroups = the grid launched, tiles = the product's C tiles (-1 before
+ * any multiply). The forms give the same bits. */
+int32_t lamb_hip_tuning_launch(const lamb_hip_ctx* ctx, int32_t* form,
                                int32_t* workgroups, int32_t* tiles);
 
 /* --- sector ownership (plexus) -------------------------------------------

@@ -182,7 +182,8 @@ __global__ __launch_bounds__(256) void pack_slices_kernel(
 // ---------------------------------------------------------------------------
 template <int S, bool kLoop>
 __global__ __launch_bounds__(256, 2) void numeric_bf16_kernel(
-    int32_t n_tiles, int32_t* __restrict__ slot_counter, const TileDesc* __restrict__ tiles,
+    int32_t n_tiles, int32_t* __restrict__ slot_counter, const int32_t* __restrict__ xrange,
+    const TileDesc* __restrict__ tiles,
     const int32_t* __restrict__ perm,
     const Entry* __restrict__ entries, const Band* __restrict__ bands,
     const uint4* __restrict__ a_sl, const uint4* __restrict__ b_sl, uint32_t a_zero,
@@ -191,14 +192,19 @@ __global__ __launch_bounds__(256, 2) void numeric_bf16_kernel(
     const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
     double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
   LAMB_GUARD;
-  // slots: blockIdx.x first. With kLoop (a persistent grid) a workgroup then
-  // takes slots x, x + 8, x + 16, ... of its residue x = blockIdx.x mod 8 (the
-  // XCD the hardware deals it to) from counter x, one claimed during each
-  // tile's epilogue: each XCD walks the same tiles, in the same order, as
-  // with one workgroup per tile. Without kLoop the body runs once, as a plain
-  // one-tile kernel with that kernel's registers.
+  // Without kLoop the body runs once, on slot blockIdx.x, as a plain one-tile
+  // kernel with that kernel's registers. With kLoop (a persistent grid) a
+  // workgroup of residue x = blockIdx.x mod 8 runs its turns j = blockIdx.x
+  // / 8, then the values of counter x, one claimed during each tile's
+  // epilogue, on the slots persistent_slot gives: interleaved (each XCD walks
+  // the tiles it walks with one workgroup per tile) or the XCD's own range.
   __shared__ int32_t next_slot;
-  for (int32_t slot = blockIdx.x;;) {
+  int32_t slot = (int32_t)blockIdx.x;
+  if constexpr (kLoop) {
+    slot = persistent_slot(xrange, n_tiles, (int32_t)(blockIdx.x >> 3));
+    if (slot < 0) return;
+  }
+  for (;;) {
     __shared__ double lds[kBandRows * kPitch];
     uint2* tbl_ab = reinterpret_cast<uint2*>(lds);  // A, B chunk
     uint32_t* tbl_m = reinterpret_cast<uint32_t*>(tbl_ab + kTableChunks);  // rmask | cmask << 16
@@ -365,8 +371,8 @@ __global__ __launch_bounds__(256, 2) void numeric_bf16_kernel(
                        kLoop ? slot_counter + (blockIdx.x & 7) : nullptr, &next_slot);
     if constexpr (!kLoop) break;
     __syncthreads();  // next_slot is visible; the next set-up rewrites the LDS
-    slot = (int32_t)(blockIdx.x & 7) + 8 * __builtin_amdgcn_readfirstlane(next_slot);
-    if (slot >= n_tiles) break;
+    slot = persistent_slot(xrange, n_tiles, __builtin_amdgcn_readfirstlane(next_slot));
+    if (slot < 0) break;
   }
 }
 
@@ -410,7 +416,7 @@ hipError_t launch_numeric_bf16(hipStream_t s, int slices, int32_t n_tiles, const
                                uint32_t b_zero, double alpha, double beta, double eps2,
                                const int64_t* old_row_p, const int32_t* old_col,
                                const int64_t* old_off, const double* old_data, double* c_data,
-                               uint16_t* kept, bool persistent, int32_t* slot_counter,
+                               uint16_t* kept, int form, int32_t* slot_counter,
                                int32_t* grid_used) {
   if (grid_used) *grid_used = 0;
   if (n_tiles == 0) return hipSuccess;
@@ -418,23 +424,29 @@ hipError_t launch_numeric_bf16(hipStream_t s, int slices, int32_t n_tiles, const
   const uint4* b = static_cast<const uint4*>(b_sl);
   auto go = [&](auto kernel) {
     int32_t grid = n_tiles;
-    if (persistent) {
+    const int32_t* xrange = nullptr;
+    if (form != kFormGrid) {
       const int32_t r = resident_workgroups(reinterpret_cast<const void*>(kernel));
       if (r > 0) grid = std::min(grid, r);
+      if (form == kFormXcdRanges) {
+        // every residue needs a workgroup, whatever the size of its range
+        grid = std::max(grid, 8);
+        xrange = slot_counter + kXcdRangeWords;
+      }
       launch_slot_counter_reset(s, slot_counter, grid);
     }
     if (grid_used) *grid_used = grid;
-    hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(256), 0, s, n_tiles, slot_counter,
+    hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(256), 0, s, n_tiles, slot_counter, xrange,
                        tiles, perm, entries, bands, a, b, a_zero, b_zero, alpha, beta, eps2,
                        old_row_p, old_col, old_off, old_data, c_data, kept);
   };
   // a persistent grid runs the looping instantiation
   switch (slices) {
-    case 3: persistent ? go(numeric_bf16_kernel<3, true>) : go(numeric_bf16_kernel<3, false>);
+    case 3: form != kFormGrid ? go(numeric_bf16_kernel<3, true>) : go(numeric_bf16_kernel<3, false>);
       break;
-    case 4: persistent ? go(numeric_bf16_kernel<4, true>) : go(numeric_bf16_kernel<4, false>);
+    case 4: form != kFormGrid ? go(numeric_bf16_kernel<4, true>) : go(numeric_bf16_kernel<4, false>);
       break;
-    case 5: persistent ? go(numeric_bf16_kernel<5, true>) : go(numeric_bf16_kernel<5, false>);
+    case 5: form != kFormGrid ? go(numeric_bf16_kernel<5, true>) : go(numeric_bf16_kernel<5, false>);
       break;
     default: return hipErrorInvalidValue;
   }

@@ -118,16 +118,30 @@ hipError_t launch_pack_strips(hipStream_t s, int64_t n_strips, const Strip* stri
 hipError_t launch_block_nsq(hipStream_t s, int64_t nblk, const int32_t* d_rows,
                             const int32_t* d_cols, const int32_t* d_blk, const int64_t* d_off,
                             const double* d_data, double* d_nsq);
+// Launch forms of the numeric kernels; every form computes every tile the
+// same way (same bits).
+// - kFormGrid: one workgroup per tile, workgroup b on position b of the tile
+//   order.
+// - kFormPersistent: the device's resident workgroups, each looping over
+//   positions x, x + 8, x + 16, ... for its residue x = blockIdx.x mod 8 (the
+//   XCD the hardware deals it to): each XCD walks the positions it walks in
+//   kFormGrid.
+// - kFormXcdRanges: resident workgroups too, residue x walking the range
+//   [xr[x], xr[x + 1]) of the order alone (xr from build_xcd_ranges), so an
+//   XCD's tiles share row and column bands in its own L2.
+// The persistent forms take their later positions from eight device
+// counters (slot_counter[0..7], reset by the launch) and read the ranges at
+// slot_counter + kXcdRangeWords; the buffer holds kSlotControlWords int32.
+constexpr int kFormGrid = 0;
+constexpr int kFormPersistent = 1;
+constexpr int kFormXcdRanges = 2;
+constexpr int kXcdRangeWords = 16;
+constexpr int kSlotControlWords = 32;
 // FP64 numeric kernel variants: per-wave 32 x 32 quadrants, per-wave 64 x 32
 // halves with two waves splitting each half's K steps, or whole-tile waves
 // splitting the K steps four ways. whole: the product has no partial entry
-// (the half and K-split kernels then run without per-lane masks). Every
-// numeric kernel runs tile slots blockIdx.x, blockIdx.x + gridDim.x, ... of
-// the tile order: persistent = false launches one workgroup per tile,
-// persistent = true the device's resident workgroups, each looping over its
-// slots (the first is blockIdx.x, later ones from slot_counter[blockIdx.x
-// mod 8], eight device int32 the launch resets); both compute every tile the
-// same way (same bits). grid_used gets the grid launched.
+// (the half and K-split kernels then run without per-lane masks). form: one
+// of the launch forms above; grid_used gets the grid launched.
 constexpr int kF64Quadrant = 0;
 constexpr int kF64Ksplit = 1;
 constexpr int kF64Half = 2;
@@ -137,7 +151,7 @@ hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* ti
                               uint32_t b_zero, double alpha, double beta, double eps2,
                               const int64_t* old_row_p, const int32_t* old_col,
                               const int64_t* old_off, const double* old_data, double* c_data,
-                              uint16_t* kept, int variant, bool whole, bool persistent,
+                              uint16_t* kept, int variant, bool whole, int form,
                               int32_t* slot_counter, int32_t* grid_used);
 // Enqueues the reset of a persistent grid's eight slot counters (one per
 // residue of blockIdx.x mod 8) for a grid of `grid` workgroups.
@@ -161,7 +175,7 @@ hipError_t launch_numeric_bf16(hipStream_t s, int slices, int32_t n_tiles, const
                                uint32_t b_zero, double alpha, double beta, double eps2,
                                const int64_t* old_row_p, const int32_t* old_col,
                                const int64_t* old_off, const double* old_data, double* c_data,
-                               uint16_t* kept, bool persistent, int32_t* slot_counter,
+                               uint16_t* kept, int form, int32_t* slot_counter,
                                int32_t* grid_used);
 hipError_t launch_trace(hipStream_t s, int32_t n_atoms, const int64_t* row_p,
                         const int32_t* col, const int64_t* off, const int32_t* blk,

@@ -82,14 +82,15 @@ struct lamb_hip_ctx {
   lamb::DevBuf d_kept, d_apack, d_bpack;
   lamb::DevBuf d_aexp, d_bexp;  // BF16 split: exponent per A row / B column
   lamb::DevBuf d_aslice, d_bslice;  // BF16 split: slice packs
-  lamb::DevBuf d_slot_counter;      // persistent numeric grids: slot counters (8)
+  lamb::DevBuf d_slot_counter;      // persistent numeric grids: slot counters, XCD ranges
   void* sym = nullptr;  // lamb::SymState*
   // autotuner: tile order (row bands per block) and launch form (one
-  // workgroup per tile, or the resident workgroups looping over tiles) per
+  // workgroup per tile, the resident workgroups looping over the order
+  // interleaved, or over one range of it per XCD: lamb::kForm*) per
   // workload class (multiply.hip)
   struct TuneChoice {
     int32_t rowblock = 8;
-    bool persistent = false;
+    int32_t form = 0;
   };
   std::map<std::array<int64_t, 4>, TuneChoice> tune;
   // tile shape (FP64 variant: 32 x 32, 64 x 32 or 64 x 64 per wave) per
@@ -99,6 +100,7 @@ struct lamb_hip_ctx {
   int32_t last_variant = -1;   // FP64 variant of the last multiply
   int32_t last_rowblock = -1;  // tile-order row bands per block of the last multiply
   int32_t last_grid = -1;      // workgroups of the last numeric launch
+  int32_t last_form = -1;      // its launch form
   int32_t last_tiles = -1;     // C tiles of the last multiply
   // Sizes the host reads back from the device (graph.hip). kSizesLive: read
   // them (stream sync); kSizesRecord: read and append to size_log;

@@ -225,7 +225,8 @@ __device__ __forceinline__ int build_step_table(const TileInfo& T, const TileSme
 
 template <bool kLoop>
 __global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
-    int32_t n_tiles, int32_t* __restrict__ slot_counter, const TileDesc* __restrict__ tiles,
+    int32_t n_tiles, int32_t* __restrict__ slot_counter, const int32_t* __restrict__ xrange,
+    const TileDesc* __restrict__ tiles,
     const int32_t* __restrict__ perm,
     const Entry* __restrict__ entries, const Band* __restrict__ bands,
     const double* __restrict__ a_pack, const double* __restrict__ b_pack, uint32_t a_zero,
@@ -234,14 +235,19 @@ __global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
     const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
     double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
   LAMB_GUARD;
-  // slots: blockIdx.x first. With kLoop (a persistent grid) a workgroup then
-  // takes slots x, x + 8, x + 16, ... of its residue x = blockIdx.x mod 8 (the
-  // XCD the hardware deals it to) from counter x, one claimed during each
-  // tile's epilogue: each XCD walks the same tiles, in the same order, as
-  // with one workgroup per tile. Without kLoop the body runs once, as a plain
-  // one-tile kernel with that kernel's registers.
+  // Without kLoop the body runs once, on slot blockIdx.x, as a plain one-tile
+  // kernel with that kernel's registers. With kLoop (a persistent grid) a
+  // workgroup of residue x = blockIdx.x mod 8 runs its turns j = blockIdx.x
+  // / 8, then the values of counter x, one claimed during each tile's
+  // epilogue, on the slots persistent_slot gives: interleaved (each XCD walks
+  // the tiles it walks with one workgroup per tile) or the XCD's own range.
   __shared__ int32_t next_slot;
-  for (int32_t slot = blockIdx.x;;) {
+  int32_t slot = (int32_t)blockIdx.x;
+  if constexpr (kLoop) {
+    slot = persistent_slot(xrange, n_tiles, (int32_t)(blockIdx.x >> 3));
+    if (slot < 0) return;
+  }
+  for (;;) {
     // main loop: the step table of a window of K columns; epilogue: C tile
     __shared__ double lds[kBandRows * kPitch];
     uint2* col_ab = reinterpret_cast<uint2*>(lds);  // A, B panel column
@@ -389,8 +395,8 @@ __global__ __launch_bounds__(256, LAMB_NUMERIC_WAVES) void numeric_f64_kernel(
                        kLoop ? slot_counter + (blockIdx.x & 7) : nullptr, &next_slot);
     if constexpr (!kLoop) break;
     __syncthreads();  // next_slot is visible; the next set-up rewrites the LDS
-    slot = (int32_t)(blockIdx.x & 7) + 8 * __builtin_amdgcn_readfirstlane(next_slot);
-    if (slot >= n_tiles) break;
+    slot = persistent_slot(xrange, n_tiles, __builtin_amdgcn_readfirstlane(next_slot));
+    if (slot < 0) break;
   }
 }
 
@@ -428,7 +434,8 @@ __device__ __forceinline__ uint32_t tile_blocks(uint32_t g) {
 // kEntryPartial), so lanes load whole panel columns without masks.
 template <bool kWhole, bool kLoop>
 __global__ __launch_bounds__(256, 2) void numeric_f64_ksplit_kernel(
-    int32_t n_tiles, int32_t* __restrict__ slot_counter, const TileDesc* __restrict__ tiles,
+    int32_t n_tiles, int32_t* __restrict__ slot_counter, const int32_t* __restrict__ xrange,
+    const TileDesc* __restrict__ tiles,
     const int32_t* __restrict__ perm,
     const Entry* __restrict__ entries, const Band* __restrict__ bands,
     const double* __restrict__ a_pack, const double* __restrict__ b_pack, uint32_t a_zero,
@@ -437,14 +444,19 @@ __global__ __launch_bounds__(256, 2) void numeric_f64_ksplit_kernel(
     const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
     double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
   LAMB_GUARD;
-  // slots: blockIdx.x first. With kLoop (a persistent grid) a workgroup then
-  // takes slots x, x + 8, x + 16, ... of its residue x = blockIdx.x mod 8 (the
-  // XCD the hardware deals it to) from counter x, one claimed during each
-  // tile's epilogue: each XCD walks the same tiles, in the same order, as
-  // with one workgroup per tile. Without kLoop the body runs once, as a plain
-  // one-tile kernel with that kernel's registers.
+  // Without kLoop the body runs once, on slot blockIdx.x, as a plain one-tile
+  // kernel with that kernel's registers. With kLoop (a persistent grid) a
+  // workgroup of residue x = blockIdx.x mod 8 runs its turns j = blockIdx.x
+  // / 8, then the values of counter x, one claimed during each tile's
+  // epilogue, on the slots persistent_slot gives: interleaved (each XCD walks
+  // the tiles it walks with one workgroup per tile) or the XCD's own range.
   __shared__ int32_t next_slot;
-  for (int32_t slot = blockIdx.x;;) {
+  int32_t slot = (int32_t)blockIdx.x;
+  if constexpr (kLoop) {
+    slot = persistent_slot(xrange, n_tiles, (int32_t)(blockIdx.x >> 3));
+    if (slot < 0) return;
+  }
+  for (;;) {
     __shared__ double lds[kBandRows * kPitch];
     uint2* col_ab = reinterpret_cast<uint2*>(lds);
     uint32_t* col_m = reinterpret_cast<uint32_t*>(col_ab + kTableCols);
@@ -586,8 +598,8 @@ __global__ __launch_bounds__(256, 2) void numeric_f64_ksplit_kernel(
                        kLoop ? slot_counter + (blockIdx.x & 7) : nullptr, &next_slot);
     if constexpr (!kLoop) break;
     __syncthreads();  // next_slot is visible; the next set-up rewrites the LDS
-    slot = (int32_t)(blockIdx.x & 7) + 8 * __builtin_amdgcn_readfirstlane(next_slot);
-    if (slot >= n_tiles) break;
+    slot = persistent_slot(xrange, n_tiles, __builtin_amdgcn_readfirstlane(next_slot));
+    if (slot < 0) break;
   }
 }
 
@@ -607,7 +619,8 @@ constexpr int kHalfPrefetch = LAMB_HALF_PREFETCH;
 
 template <bool kWhole, bool kLoop>
 __global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
-    int32_t n_tiles, int32_t* __restrict__ slot_counter, const TileDesc* __restrict__ tiles,
+    int32_t n_tiles, int32_t* __restrict__ slot_counter, const int32_t* __restrict__ xrange,
+    const TileDesc* __restrict__ tiles,
     const int32_t* __restrict__ perm,
     const Entry* __restrict__ entries, const Band* __restrict__ bands,
     const double* __restrict__ a_pack, const double* __restrict__ b_pack, uint32_t a_zero,
@@ -616,14 +629,19 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
     const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
     double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
   LAMB_GUARD;
-  // slots: blockIdx.x first. With kLoop (a persistent grid) a workgroup then
-  // takes slots x, x + 8, x + 16, ... of its residue x = blockIdx.x mod 8 (the
-  // XCD the hardware deals it to) from counter x, one claimed during each
-  // tile's epilogue: each XCD walks the same tiles, in the same order, as
-  // with one workgroup per tile. Without kLoop the body runs once, as a plain
-  // one-tile kernel with that kernel's registers.
+  // Without kLoop the body runs once, on slot blockIdx.x, as a plain one-tile
+  // kernel with that kernel's registers. With kLoop (a persistent grid) a
+  // workgroup of residue x = blockIdx.x mod 8 runs its turns j = blockIdx.x
+  // / 8, then the values of counter x, one claimed during each tile's
+  // epilogue, on the slots persistent_slot gives: interleaved (each XCD walks
+  // the tiles it walks with one workgroup per tile) or the XCD's own range.
   __shared__ int32_t next_slot;
-  for (int32_t slot = blockIdx.x;;) {
+  int32_t slot = (int32_t)blockIdx.x;
+  if constexpr (kLoop) {
+    slot = persistent_slot(xrange, n_tiles, (int32_t)(blockIdx.x >> 3));
+    if (slot < 0) return;
+  }
+  for (;;) {
     __shared__ double lds[kBandRows * kPitch];
     uint2* col_ab = reinterpret_cast<uint2*>(lds);
     uint32_t* col_m = reinterpret_cast<uint32_t*>(col_ab + kTableCols);
@@ -771,8 +789,8 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
                        kLoop ? slot_counter + (blockIdx.x & 7) : nullptr, &next_slot);
     if constexpr (!kLoop) break;
     __syncthreads();  // next_slot is visible; the next set-up rewrites the LDS
-    slot = (int32_t)(blockIdx.x & 7) + 8 * __builtin_amdgcn_readfirstlane(next_slot);
-    if (slot >= n_tiles) break;
+    slot = persistent_slot(xrange, n_tiles, __builtin_amdgcn_readfirstlane(next_slot));
+    if (slot < 0) break;
   }
 }
 
@@ -798,10 +816,10 @@ int32_t resident_workgroups(const void* kernel) {
   return n;
 }
 
-// Sets a persistent grid's eight slot counters: counter x counts the slots
-// x + 8k already taken by the workgroups of residue x (blockIdx.x mod 8),
-// which take slots x, x + 8, ... below the grid size first. A kernel, not a
-// memset, so that graph captures replay it.
+// Sets a persistent grid's eight slot counters: counter x counts the turns
+// already taken by the workgroups of residue x (blockIdx.x mod 8), whose
+// first turns are 0, 1, ... (blockIdx.x / 8), one per workgroup. A kernel,
+// not a memset, so that graph captures replay it.
 __global__ void slot_counter_reset(int32_t* __restrict__ counter, int32_t grid) {
   LAMB_GUARD;
   const int x = threadIdx.x;
@@ -818,38 +836,44 @@ hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* ti
                               uint32_t b_zero, double alpha, double beta, double eps2,
                               const int64_t* old_row_p, const int32_t* old_col,
                               const int64_t* old_off, const double* old_data, double* c_data,
-                              uint16_t* kept, int variant, bool whole, bool persistent,
+                              uint16_t* kept, int variant, bool whole, int form,
                               int32_t* slot_counter, int32_t* grid_used) {
   if (grid_used) *grid_used = 0;
   if (n_tiles == 0) return hipSuccess;
   auto go = [&](auto kernel) {
     int32_t grid = n_tiles;
-    if (persistent) {
+    const int32_t* xrange = nullptr;
+    if (form != kFormGrid) {
       const int32_t r = resident_workgroups(reinterpret_cast<const void*>(kernel));
       if (r > 0) grid = std::min(grid, r);
+      if (form == kFormXcdRanges) {
+        // every residue needs a workgroup, whatever the size of its range
+        grid = std::max(grid, 8);
+        xrange = slot_counter + kXcdRangeWords;
+      }
       launch_slot_counter_reset(s, slot_counter, grid);
     }
     if (grid_used) *grid_used = grid;
-    hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(256), 0, s, n_tiles, slot_counter,
+    hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(256), 0, s, n_tiles, slot_counter, xrange,
                        tiles, perm, entries, bands, a_pack, b_pack, a_zero, b_zero, alpha, beta,
                        eps2, old_row_p, old_col, old_off, old_data, c_data, kept);
   };
   // a persistent grid runs the looping instantiation; one workgroup per tile
   // the straight one, whose registers are those of a single tile
   if (variant == kF64Ksplit && whole)
-    persistent ? go(numeric_f64_ksplit_kernel<true, true>)
+    form != kFormGrid ? go(numeric_f64_ksplit_kernel<true, true>)
                : go(numeric_f64_ksplit_kernel<true, false>);
   else if (variant == kF64Ksplit)
-    persistent ? go(numeric_f64_ksplit_kernel<false, true>)
+    form != kFormGrid ? go(numeric_f64_ksplit_kernel<false, true>)
                : go(numeric_f64_ksplit_kernel<false, false>);
   else if (variant == kF64Half && whole)
-    persistent ? go(numeric_f64_half_kernel<true, true>)
+    form != kFormGrid ? go(numeric_f64_half_kernel<true, true>)
                : go(numeric_f64_half_kernel<true, false>);
   else if (variant == kF64Half)
-    persistent ? go(numeric_f64_half_kernel<false, true>)
+    form != kFormGrid ? go(numeric_f64_half_kernel<false, true>)
                : go(numeric_f64_half_kernel<false, false>);
   else
-    persistent ? go(numeric_f64_kernel<true>) : go(numeric_f64_kernel<false>);
+    form != kFormGrid ? go(numeric_f64_kernel<true>) : go(numeric_f64_kernel<false>);
   return hipGetLastError();
 }
 

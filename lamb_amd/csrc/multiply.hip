@@ -286,8 +286,10 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
   // shape) comes from the rule unless shape tuning is on or a loaded table
   // names it: variants add K products in different orders, so only an
   // explicit request lets timings choose it. The autotuner then times the
-  // candidate tile orders, each with one workgroup per tile and with a
-  // persistent grid, on the first product of a workload class (form,
+  // candidate tile orders, each in the three launch forms (one workgroup per
+  // tile, a persistent grid over the interleaved order, a persistent grid
+  // with one range of the order per XCD), on the first product of a workload
+  // class (form,
   // variant, largest block, half-octave buckets of flops per tile and of C
   // blocks per tile) and keeps the fastest for the context's lifetime; a
   // graph capture with no tuned class uses 8 row bands per block and one
@@ -321,7 +323,7 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
   const char* env_pe = getenv("LAMB_HIP_PERSISTENT");
   if (env_rb || env_pe) {
     if (env_rb) choice.rowblock = std::max(1, std::min(atoi(env_rb), 32));
-    if (env_pe) choice.persistent = atoi(env_pe) != 0;
+    if (env_pe) choice.form = std::max(kFormGrid, std::min(atoi(env_pe), kFormXcdRanges));
   }
   HIPCHK(ctx, build_tile_order(ctx, S.W, D, S.r, choice.rowblock), "tile order");
   HIPCHK(ctx, record_event(ctx, 3), "event");
@@ -335,10 +337,10 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
   C->s_data = DevBuf();
   HIPCHK(ctx, newdata.ensure(sizeof(double) * (size_t)std::max<int64_t>(S.r.c_size, 1)),
          "C pool");
-  HIPCHK(ctx, ctx->d_slot_counter.ensure(8 * sizeof(int32_t)), "slot counters");
+  HIPCHK(ctx, ctx->d_slot_counter.ensure(kSlotControlWords * sizeof(int32_t)), "slot counters");
   int32_t grid = 0;
   int32_t* slot_counter = (int32_t*)ctx->d_slot_counter.p;
-  auto launch_numeric = [&](bool persistent) -> hipError_t {
+  auto launch_numeric = [&](int form) -> hipError_t {
     if (!split)
       return launch_numeric_f64(s, n_tiles, (const TileDesc*)S.r.tiles.p,
                                 (const int32_t*)S.r.perm.p, (const Entry*)S.r.entries.p, D.bands,
@@ -348,22 +350,22 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
                                 use_old ? MC.off : nullptr,
                                 use_old ? (const double*)C->d_data.p : nullptr,
                                 (double*)newdata.p, (uint16_t*)ctx->d_kept.p, variant, whole,
-                                persistent, slot_counter, &grid);
+                                form, slot_counter, &grid);
     return launch_numeric_bf16(s, slices, n_tiles, (const TileDesc*)S.r.tiles.p,
                                (const int32_t*)S.r.perm.p, (const Entry*)S.r.entries.p, D.bands,
                                ctx->d_aslice.p, ctx->d_bslice.p, a_zero, b_zero, alpha, beta,
                                post2, use_old ? MC.row_p : nullptr, use_old ? MC.col : nullptr,
                                use_old ? MC.off : nullptr,
                                use_old ? (const double*)C->d_data.p : nullptr,
-                               (double*)newdata.p, (uint16_t*)ctx->d_kept.p, persistent,
+                               (double*)newdata.p, (uint16_t*)ctx->d_kept.p, form,
                                slot_counter, &grid);
   };
   // time one launch of the numeric kernel (every candidate writes the whole
   // C pool and kept masks; the last launch's result is the call's)
-  auto timed = [&](bool persistent, float* ms) -> hipError_t {
+  auto timed = [&](int form, float* ms) -> hipError_t {
     hipError_t e;
     if ((e = record_event(ctx, 0)) != hipSuccess) return e;
-    if ((e = launch_numeric(persistent)) != hipSuccess) return e;
+    if ((e = launch_numeric(form)) != hipSuccess) return e;
     if ((e = record_event(ctx, 1)) != hipSuccess) return e;
     if ((e = hipEventSynchronize(ctx->ev[1])) != hipSuccess) return e;
     return hipEventElapsedTime(ms, ctx->ev[0], ctx->ev[1]);
@@ -376,7 +378,7 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
     for (int v : {kF64Half, kF64Quadrant, kF64Ksplit}) {
       variant = v;
       float ms = 0.f;
-      HIPCHK(ctx, timed(choice.persistent, &ms), "numeric kernel (shape tuning)");
+      HIPCHK(ctx, timed(choice.form, &ms), "numeric kernel (shape tuning)");
       if (v == kF64Half || ms < best) {
         best = ms;
         best_v = v;
@@ -407,13 +409,14 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
     bool first = true;
     for (size_t c = 0; c < sizeof(kRowBlocks) / sizeof(kRowBlocks[0]); ++c) {
       HIPCHK(ctx, build_tile_order(ctx, S.W, D, S.r, kRowBlocks[c]), "tile order");
-      for (int pers = 0; pers < 2; ++pers) {
+      HIPCHK(ctx, build_xcd_ranges(ctx, S.W, S.r, slot_counter + kXcdRangeWords), "XCD ranges");
+      for (int form = kFormGrid; form <= kFormXcdRanges; ++form) {
         float ms = 0.f;
-        HIPCHK(ctx, timed(pers != 0, &ms), "numeric kernel (tuning)");
+        HIPCHK(ctx, timed(form, &ms), "numeric kernel (tuning)");
         if (first || ms < best) {
           best = ms;
           best_choice.rowblock = kRowBlocks[c];
-          best_choice.persistent = pers != 0;
+          best_choice.form = form;
           best_grid = grid;
         }
         first = false;
@@ -423,13 +426,16 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
     choice = best_choice;
     grid = best_grid;
   } else {
+    if (choice.form == kFormXcdRanges)
+      HIPCHK(ctx, build_xcd_ranges(ctx, S.W, S.r, slot_counter + kXcdRangeWords), "XCD ranges");
     HIPCHK(ctx, record_event(ctx, 0), "event");
-    HIPCHK(ctx, launch_numeric(choice.persistent),
+    HIPCHK(ctx, launch_numeric(choice.form),
            split ? "numeric kernel (BF16 split)" : "numeric kernel");
     HIPCHK(ctx, record_event(ctx, 1), "event");
   }
   ctx->last_rowblock = choice.rowblock;
   ctx->last_grid = grid;
+  ctx->last_form = choice.form;
   ctx->last_tiles = n_tiles;
 
   // ---- C index ----
@@ -657,7 +663,7 @@ extern "C" int32_t lamb_hip_tuning_mode(lamb_hip_ctx* ctx, int32_t tune_shapes) 
 
 // Tuning table file: a header line, then one line per tuned class.
 //   shape k0 k1 k2 k3 k4 variant
-//   order k0 k1 k2 k3 rowblock persistent
+//   order k0 k1 k2 k3 rowblock form
 static const char kTuneHeader[] = "lamb_hip tuning table 1";
 
 extern "C" int32_t lamb_hip_tuning_save(const lamb_hip_ctx* ctx, const char* path) {
@@ -674,7 +680,7 @@ extern "C" int32_t lamb_hip_tuning_save(const lamb_hip_ctx* ctx, const char* pat
             (long long)k[2], (long long)k[3], (long long)k[4], v);
   for (const auto& [k, v] : ctx->tune)
     fprintf(f, "order %lld %lld %lld %lld %d %d\n", (long long)k[0], (long long)k[1],
-            (long long)k[2], (long long)k[3], v.rowblock, v.persistent ? 1 : 0);
+            (long long)k[2], (long long)k[3], v.rowblock, v.form);
   const bool ok = fclose(f) == 0;
   return ok ? LAMB_HIP_OK
             : set_error(c, LAMB_HIP_FAILURE, LAMB_HIP_EXTERNAL_ERROR,
@@ -712,10 +718,10 @@ extern "C" int32_t lamb_hip_tuning_load(lamb_hip_ctx* ctx, const char* path) {
       shapes[{k[0], k[1], k[2], k[3], k[4]}] = a;
     } else if (sscanf(line, "order %lld %lld %lld %lld %d %d %c", &k[0], &k[1], &k[2], &k[3], &a,
                       &b, &tail) == 6 &&
-               a >= 1 && a <= 32 && (b == 0 || b == 1)) {
+               a >= 1 && a <= 32 && b >= kFormGrid && b <= kFormXcdRanges) {
       lamb_hip_ctx::TuneChoice t;
       t.rowblock = a;
-      t.persistent = b != 0;
+      t.form = b;
       orders[{k[0], k[1], k[2], k[3]}] = t;
     } else {
       rc = set_error(ctx, LAMB_HIP_FAILURE, LAMB_HIP_WRONG_ARGS,
@@ -732,10 +738,10 @@ extern "C" int32_t lamb_hip_tuning_load(lamb_hip_ctx* ctx, const char* path) {
   return LAMB_HIP_OK;
 }
 
-extern "C" int32_t lamb_hip_tuning_launch(const lamb_hip_ctx* ctx, int32_t* persistent,
+extern "C" int32_t lamb_hip_tuning_launch(const lamb_hip_ctx* ctx, int32_t* form,
                                           int32_t* workgroups, int32_t* tiles) {
   if (!ctx) return LAMB_HIP_FAILURE;
-  if (persistent) *persistent = ctx->last_grid >= 0 && ctx->last_grid < ctx->last_tiles;
+  if (form) *form = ctx->last_form;
   if (workgroups) *workgroups = ctx->last_grid;
   if (tiles) *tiles = ctx->last_tiles;
   return LAMB_HIP_OK;

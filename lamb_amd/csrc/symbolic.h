@@ -48,10 +48,10 @@ struct StripSet {
 
 struct SymScratch {
   DevBuf merged, flag, excl, scan, gslot;
-  DevBuf cnt, ksum, pat, stats, band4, band4s, cells, cellcnt, cellbase, mpat, cidx;
+  DevBuf cnt, ksum, pat, stats, band4, band4s, cells, cellcnt, cellbase, mpat, cidx, tw;
   void release() {
     for (DevBuf* b : {&merged, &flag, &excl, &scan, &gslot, &cnt, &ksum, &pat, &stats, &band4,
-                      &band4s, &cells, &cellcnt, &cellbase, &mpat, &cidx})
+                      &band4s, &cells, &cellcnt, &cellbase, &mpat, &cidx, &tw})
       b->release();
   }
 };
@@ -85,6 +85,10 @@ hipError_t build_tiles(lamb_hip_ctx* ctx, SymScratch& W, const DistView& D, cons
 // column band by column band)
 hipError_t build_tile_order(lamb_hip_ctx* ctx, SymScratch& W, const DistView& D, SymResult& r,
                             int rb);
+// xr[0..8]: the XCD ranges of the tile order perm. Residue x of a persistent
+// grid walks positions [xr[x], xr[x + 1]), one contiguous eighth of the
+// order by weight (each tile weighs its K columns, ktot).
+hipError_t build_xcd_ranges(lamb_hip_ctx* ctx, SymScratch& W, const SymResult& r, int32_t* xr);
 hipError_t build_c_index(lamb_hip_ctx* ctx, SymScratch& W, const DistView& D,
                          const SymResult& r, const uint16_t* kept, CIndex& c);
 

@@ -848,6 +848,37 @@ __global__ void tile_perm(int64_t ntiles, int rb, int32_t nbands, const TileDesc
   perm[base[c] + __builtin_popcount(below)] = (int32_t)t;
 }
 
+// XCD ranges: weight of each position of the tile order (its tile's K
+// columns, which track the tile's loads and MFMAs to a few per cent), then,
+// from the exclusive scan acc of the weights (acc[nt] = total), position p
+// goes to XCD floor(8 (acc[p] + w_p / 2) / total). That XCD index never
+// decreases along the order, so each XCD gets one contiguous range, and
+// xr[x] = its first position: p writes xr[x] for the XCDs x that start at p
+// (those after its predecessor's, up to its own). xr holds nt beforehand, so
+// XCDs past the last position get the empty range [nt, nt).
+__global__ void tile_weights(int64_t nt, const TileDesc* __restrict__ tiles,
+                             const int32_t* __restrict__ perm, int64_t* __restrict__ w) {
+  LAMB_GUARD;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < nt) w[p] = tiles[perm[p]].ktot;
+}
+
+__device__ __forceinline__ int xcd_of_position(const int64_t* __restrict__ acc, int64_t nt,
+                                               int64_t p) {
+  const int64_t tot = acc[nt], a = acc[p], w = acc[p + 1] - a;
+  const int64_t x = tot > 0 ? (8 * (2 * a + w)) / (2 * tot) : (8 * p) / nt;
+  return (int)(x < 7 ? x : 7);
+}
+
+__global__ void xcd_bounds(int64_t nt, const int64_t* __restrict__ acc, int32_t* __restrict__ xr) {
+  LAMB_GUARD;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nt) return;
+  const int xp = xcd_of_position(acc, nt, p);
+  const int xq = p == 0 ? -1 : xcd_of_position(acc, nt, p - 1);
+  for (int x = xq + 1; x <= xp; ++x) xr[x] = (int32_t)p;
+}
+
 // out[0..3] = the four chunk-total sums (after their scans, at [v*(nb+2)+nb]
 // with nb the number of scanned values), out[4..7] = kept, dropped, flops
 // and partial-entry statistics
@@ -1100,6 +1131,22 @@ hipError_t build_tile_order(lamb_hip_ctx* ctx, SymScratch& W, const DistView& D,
   hipLaunchKernelGGL(tile_perm, dim3(grid_of(nt)), dim3(256), 0, s, nt, rb, nb,
                      (const TileDesc*)r.tiles.p, (const uint32_t*)W.cells.p,
                      (const int64_t*)W.cellbase.p, (int32_t*)r.perm.p);
+  return hipGetLastError();
+}
+
+hipError_t build_xcd_ranges(lamb_hip_ctx* ctx, SymScratch& W, const SymResult& r, int32_t* xr) {
+  hipStream_t s = ctx->stream;
+  const int64_t nt = r.n_tiles;
+  CK(W.tw.ensure(sizeof(int64_t) * (size_t)(nt + 1)));
+  CK(W.scan.ensure(sizeof(int64_t) * (size_t)scan_scratch_elems(nt + 1)));
+  CK(fill32(s, xr, 9, (int32_t)nt));
+  if (nt == 0) return hipSuccess;
+  hipLaunchKernelGGL(tile_weights, dim3(grid_of(nt)), dim3(256), 0, s, nt,
+                     (const TileDesc*)r.tiles.p, (const int32_t*)r.perm.p, (int64_t*)W.tw.p);
+  CK(hipGetLastError());
+  CK(exclusive_scan(s, (const int64_t*)W.tw.p, (int64_t*)W.tw.p, nt, (int64_t*)W.scan.p));
+  hipLaunchKernelGGL(xcd_bounds, dim3(grid_of(nt)), dim3(256), 0, s, nt, (const int64_t*)W.tw.p,
+                     xr);
   return hipGetLastError();
 }
 

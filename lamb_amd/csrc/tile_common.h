@@ -23,6 +23,23 @@ struct TileSmem {
   int16_t exact_id[kBandAtoms * kBandAtoms];
 };
 
+// Tile slot (position in the tile order) of the j-th tile of a persistent
+// workgroup of residue x = blockIdx.x mod 8, the XCD the hardware deals it
+// to. Interleaved (xrange null): slot x + 8 j, below n_tiles, so the XCDs
+// share one window of the order. XCD ranges: position xrange[x] + j of the
+// range [xrange[x], xrange[x + 1]) that residue x walks alone. -1 past the
+// end of the workgroup's sequence.
+__device__ __forceinline__ int32_t persistent_slot(const int32_t* __restrict__ xrange,
+                                                   int32_t n_tiles, int32_t j) {
+  const int32_t x = (int32_t)(blockIdx.x & 7);
+  if (xrange != nullptr) {
+    const int32_t s = xrange[x] + j;
+    return s < xrange[x + 1] ? s : -1;
+  }
+  const int32_t s = x + 8 * j;
+  return s < n_tiles ? s : -1;
+}
+
 // v of the lane a DPP control selects (0 where it selects none), for a
 // 64-bit value as two 32-bit moves
 template <int kCtrl, int kRowMask>

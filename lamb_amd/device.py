@@ -254,11 +254,13 @@ def tuning_order(ctx: Context) -> int:
 
 
 def tuning_launch(ctx: Context):
-    """(persistent?, workgroups launched, C tiles) of the last numeric kernel."""
+    """(launch form, workgroups launched, C tiles) of the last numeric kernel.
+    Forms: 0 one workgroup per tile, 1 persistent grid over the interleaved
+    tile order, 2 persistent grid with one range of the order per XCD."""
     p, w, t = C.c_int32(), C.c_int32(), C.c_int32()
     ctx.check(ctx.lib.lamb_hip_tuning_launch(ctx.h, C.byref(p), C.byref(w), C.byref(t)),
               "tuning_launch")
-    return bool(p.value), w.value, t.value
+    return p.value, w.value, t.value
 
 
 def tuning_mode(ctx: Context, tune_shapes: bool) -> None:

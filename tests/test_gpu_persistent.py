@@ -1,7 +1,8 @@
-"""Persistent launch form of the numeric kernels: a grid of the device's
-resident workgroups, each looping over tile slots of the tile order, in
-place of one workgroup per tile. Every tile is computed the same way in both
-forms, so each kernel variant must give the same bits either way."""
+"""Persistent launch forms of the numeric kernels: a grid of the device's
+resident workgroups, each looping over tile slots of the tile order, in place
+of one workgroup per tile; the XCDs either share one window of the order or
+each walk their own range of it. Every tile is computed the same way in all
+forms, so each kernel variant must give the same bits in each."""
 import numpy as np
 import pytest
 
@@ -39,28 +40,49 @@ def cases():
 
 
 @pytest.mark.parametrize("variant", ["quadrant", "half", "ksplit", "bf16"])
-def test_persistent_grid_gives_the_same_bits(ctx, monkeypatch, variant):
+def test_persistent_forms_give_the_same_bits(ctx, monkeypatch, variant):
     flags = D.BF16_SPLIT if variant == "bf16" else D.FP64
     if variant != "bf16":
         monkeypatch.setenv("LAMB_HIP_F64_KERNEL", variant)
     for name, atoms, a, b, eps in cases():
         print(f"case: {variant} {name} atoms={atoms.n_atoms} eps={eps}", flush=True)
         monkeypatch.setenv("LAMB_HIP_PERSISTENT", "0")
-        g0, st0, (p0, w0, t0) = run(ctx, atoms, a, b, eps, flags)
-        monkeypatch.setenv("LAMB_HIP_PERSISTENT", "1")
-        g1, st1, (p1, w1, t1) = run(ctx, atoms, a, b, eps, flags)
-        print(f"  grid {w0} workgroups for {t0} tiles; persistent {w1}", flush=True)
-        assert not p0 and w0 == t0
-        assert p1 and 0 < w1 < t1 == t0  # the resident workgroups loop
-        assert np.array_equal(g0.row_p, g1.row_p) and np.array_equal(g0.col, g1.col)
-        assert np.array_equal(g0.data, g1.data)
-        assert st0["n_triples_kept"] == st1["n_triples_kept"]
+        g0, st0, (f0, w0, t0) = run(ctx, atoms, a, b, eps, flags)
+        assert f0 == 0 and w0 == t0
+        for form in (1, 2):
+            monkeypatch.setenv("LAMB_HIP_PERSISTENT", str(form))
+            g1, st1, (f1, w1, t1) = run(ctx, atoms, a, b, eps, flags)
+            print(f"  form {form}: {w1} workgroups for {t1} tiles (grid form {w0})", flush=True)
+            assert f1 == form and 0 < w1 < t1 == t0  # the resident workgroups loop
+            assert np.array_equal(g0.row_p, g1.row_p) and np.array_equal(g0.col, g1.col)
+            assert np.array_equal(g0.data, g1.data)
+            assert st0["n_triples_kept"] == st1["n_triples_kept"]
         if name == "partial":
             assert st0["n_triples_filtered"] > 0
         ref, rst = oracle.multiply(a, b, eps=eps)
         same, err = compare(g1, ref)
         assert same and st1["n_triples_kept"] == rst["n_triples_kept"]
         assert err <= (1e-9 if variant == "bf16" else 1e-12), err
+
+
+@pytest.mark.parametrize("n_atoms", [3, 20, 60])
+def test_xcd_ranges_with_fewer_tiles_than_xcds(ctx, monkeypatch, n_atoms):
+    """Products of 1 to a few tiles: most XCD ranges are empty, and the grid
+    is raised to one workgroup per residue."""
+    atoms = S.water_system(n_atoms, seed=221)
+    a = S.decay_matrix(atoms, 8.0, 0.4, 222)
+    b = S.decay_matrix(atoms, 8.0, 0.4, 223)
+    print(f"case: {atoms.n_atoms} atoms", flush=True)
+    monkeypatch.setenv("LAMB_HIP_PERSISTENT", "0")
+    g0, _, (_, _, t0) = run(ctx, atoms, a, b, 1e-9, D.FP64)
+    monkeypatch.setenv("LAMB_HIP_PERSISTENT", "2")
+    g2, _, (f2, w2, t2) = run(ctx, atoms, a, b, 1e-9, D.FP64)
+    print(f"  {t2} tiles, {w2} workgroups", flush=True)
+    assert f2 == 2 and t2 == t0 and w2 >= 8
+    assert np.array_equal(g0.data, g2.data) and np.array_equal(g0.col, g2.col)
+    ref, _ = oracle.multiply(a, b, eps=1e-9)
+    same, err = compare(g2, ref)
+    assert same and err <= 1e-12, err
 
 
 def test_autotuner_keeps_a_launch_form(ctx):
@@ -73,7 +95,7 @@ def test_autotuner_keeps_a_launch_form(ctx):
     try:
         g1, _, l1 = run(fresh, atoms, a, b, 1e-7, D.FP64)
         g2, _, l2 = run(fresh, atoms, a, b, 1e-7, D.FP64)
-        print(f"tuned launch: persistent={l1[0]} workgroups={l1[1]} tiles={l1[2]}", flush=True)
+        print(f"tuned launch: form={l1[0]} workgroups={l1[1]} tiles={l1[2]}", flush=True)
         assert l1 == l2 and D.tuning_info(fresh)[1] == 1
         assert np.array_equal(g1.data, g2.data)
     finally:

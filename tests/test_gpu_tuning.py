@@ -93,7 +93,7 @@ def check_refusals(ctx, tmp_path):
         "header": "not a table\nshape 0 0 0 0 0 2\n",
         "variant": "lamb_hip tuning table 1\nshape 0 1 2 3 4 7\n",
         "rowblock": "lamb_hip tuning table 1\norder 0 1 2 3 0 0\n",
-        "persistent": "lamb_hip tuning table 1\norder 0 1 2 3 8 2\n",
+        "form": "lamb_hip tuning table 1\norder 0 1 2 3 8 3\n",
         "fields": "lamb_hip tuning table 1\nshape 0 1 2 3 2\n",
         "trailing": "lamb_hip tuning table 1\norder 0 1 2 3 8 0 9\n",
         "empty": "",
