@@ -30,10 +30,13 @@ PEAK_HBM_GBS = 8000.0
 PROFILED_TRAFFIC_BYTES = {
     ("numeric_f64_half_kernel", 4, 0): 109.94e9,
     ("numeric_f64_half_kernel", 4, 1): 117.64e9,
+    ("numeric_f64_half_kernel", 4, 2): 87.37e9,
     ("numeric_f64_half_kernel", 8, 0): 108.31e9,
     ("numeric_f64_half_kernel", 8, 1): 114.90e9,
+    ("numeric_f64_half_kernel", 8, 2): 97.30e9,
     ("numeric_f64_half_kernel", 16, 0): 121.87e9,
     ("numeric_f64_half_kernel", 16, 1): 128.66e9,
+    ("numeric_f64_half_kernel", 16, 2): 106.19e9,
 }
 LAUNCH_FORMS = {0: "one workgroup per tile", 1: "persistent grid, XCDs share one window",
                 2: "persistent grid, one range of the tile order per XCD"}
