@@ -340,7 +340,13 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
   HIPCHK(ctx, ctx->d_slot_counter.ensure(kSlotControlWords * sizeof(int32_t)), "slot counters");
   int32_t grid = 0;
   int32_t* slot_counter = (int32_t*)ctx->d_slot_counter.p;
+  // every launch in the XCD ranges form first splits the current tile order
+  // (three small kernels), so the bounds always match the order it walks
   auto launch_numeric = [&](int form) -> hipError_t {
+    if (form == kFormXcdRanges) {
+      const hipError_t e = build_xcd_ranges(ctx, S.W, S.r, slot_counter + kXcdRangeWords);
+      if (e != hipSuccess) return e;
+    }
     if (!split)
       return launch_numeric_f64(s, n_tiles, (const TileDesc*)S.r.tiles.p,
                                 (const int32_t*)S.r.perm.p, (const Entry*)S.r.entries.p, D.bands,
@@ -409,7 +415,6 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
     bool first = true;
     for (size_t c = 0; c < sizeof(kRowBlocks) / sizeof(kRowBlocks[0]); ++c) {
       HIPCHK(ctx, build_tile_order(ctx, S.W, D, S.r, kRowBlocks[c]), "tile order");
-      HIPCHK(ctx, build_xcd_ranges(ctx, S.W, S.r, slot_counter + kXcdRangeWords), "XCD ranges");
       for (int form = kFormGrid; form <= kFormXcdRanges; ++form) {
         float ms = 0.f;
         HIPCHK(ctx, timed(form, &ms), "numeric kernel (tuning)");
@@ -426,8 +431,6 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
     choice = best_choice;
     grid = best_grid;
   } else {
-    if (choice.form == kFormXcdRanges)
-      HIPCHK(ctx, build_xcd_ranges(ctx, S.W, S.r, slot_counter + kXcdRangeWords), "XCD ranges");
     HIPCHK(ctx, record_event(ctx, 0), "event");
     HIPCHK(ctx, launch_numeric(choice.form),
            split ? "numeric kernel (BF16 split)" : "numeric kernel");

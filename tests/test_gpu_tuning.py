@@ -113,3 +113,23 @@ def check_refusals(ctx, tmp_path):
     with pytest.raises(D.LambError) as err:
         ctx.check(ctx.lib.lamb_hip_tuning_mode(ctx.h, 2), "tuning_mode")
     assert err.value.type == 100
+
+
+def test_shape_tuning_in_the_xcd_ranges_form(monkeypatch):
+    """Shape tuning launches each FP64 shape in the forced launch form; in the
+    XCD ranges form each of those launches splits the tile order first."""
+    monkeypatch.setenv("LAMB_HIP_PERSISTENT", "2")
+    ctx = D.Context(0)
+    try:
+        D.tuning_mode(ctx, True)
+        for atoms, a, b, eps in products():
+            got, st, variant = run(ctx, atoms, a, b, eps)
+            form = D.tuning_launch(ctx)[0]
+            print(f"case: atoms={atoms.n_atoms} eps={eps} shape={variant} form={form}", flush=True)
+            assert form == 2
+            ref, rst = oracle.multiply(a, b, eps=eps)
+            same, err = compare(got, ref)
+            assert same and err <= 1e-12, err
+            assert st["n_triples_kept"] == rst["n_triples_kept"]
+    finally:
+        ctx.close()
