@@ -221,8 +221,7 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
 
   HIPCHK(ctx, record_event(ctx, 2), "event");
   // ---- strips and packed panels ----
-  HIPCHK(ctx, build_strips(ctx, S.W, MA, D, true, S.sa), "A strips");
-  HIPCHK(ctx, build_strips(ctx, S.W, MB, D, false, S.sb), "B strips");
+  HIPCHK(ctx, build_strips(ctx, S.W, MA, MB, D, S.sa, S.sb), "strips");
   // panel columns are addressed by 32-bit indices (31 bits on the A side,
   // whose top bit flags partial entries); one zero column follows the last
   // panel of each side

@@ -45,9 +45,11 @@ __device__ int64_t block_exclusive_scan(int64_t v, int64_t* total) {
   return base + x - v;
 }
 
-// out[i] = sum of in[0..i) within each tile; tile totals to sums[blockIdx]
-__global__ __launch_bounds__(kScanBlock) void scan_tiles(const int64_t* __restrict__ in,
-                                                         int64_t* __restrict__ out, int64_t n,
+// out[i] = sum of in[0..i) within each tile; tile totals to sums[blockIdx].
+// in and out may be the same array (no __restrict__ on them): every thread
+// reads its elements before the block scan's barriers and writes after.
+__global__ __launch_bounds__(kScanBlock) void scan_tiles(const int64_t* in, int64_t* out,
+                                                         int64_t n,
                                                          int64_t* __restrict__ sums) {
   LAMB_GUARD;
   const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
@@ -77,6 +79,34 @@ __global__ __launch_bounds__(kScanBlock) void add_tile_base(int64_t* __restrict_
   for (int i = 0; i < kScanItems; ++i)
     if (base + i < n) out[base + i] += add;
 }
+// Arrays of at most kSmallScan values: one workgroup walks the tiles in
+// order with a running carry and writes out[n] = total, one launch in place
+// of the three phases' five. in and out may be the same array, as above.
+constexpr int64_t kSmallScan = 8 * (int64_t)kScanTile;
+__global__ __launch_bounds__(kScanBlock) void scan_small(const int64_t* in, int64_t* out,
+                                                         int64_t n) {
+  LAMB_GUARD;
+  int64_t carry = 0;
+  for (int64_t t0 = 0; t0 < n; t0 += kScanTile) {
+    const int64_t base = t0 + (int64_t)threadIdx.x * kScanItems;
+    int64_t v[kScanItems];
+    int64_t t = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+      v[i] = base + i < n ? in[base + i] : 0;
+      t += v[i];
+    }
+    int64_t total;
+    int64_t run = carry + block_exclusive_scan(t, &total);
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+      if (base + i < n) out[base + i] = run;
+      run += v[i];
+    }
+    carry += total;
+  }
+  if (threadIdx.x == 0) out[n] = carry;
+}
 }  // namespace
 
 // Exclusive scan of in[0..n) into out[0..n]; out[n] = total. in and out may
@@ -95,6 +125,10 @@ hipError_t exclusive_scan(hipStream_t s, const int64_t* in, int64_t* out, int64_
                           int64_t* scratch) {
   if (n <= 0) {
     hipLaunchKernelGGL(fill_i64, dim3(1), dim3(64), 0, s, out, (int64_t)1, (int64_t)0);
+    return hipGetLastError();
+  }
+  if (n <= kSmallScan) {
+    hipLaunchKernelGGL(scan_small, dim3(1), dim3(kScanBlock), 0, s, in, out, n);
     return hipGetLastError();
   }
   const int64_t tiles = (n + kScanTile - 1) / kScanTile;

@@ -49,9 +49,11 @@ struct StripSet {
 struct SymScratch {
   DevBuf merged, flag, excl, scan, gslot;
   DevBuf cnt, ksum, pat, stats, band4, band4s, cells, cellcnt, cellbase, mpat, cidx, tw;
+  DevBuf flag_b, excl_b, gslot_b, sizes;  // B side of the strip build; counts read back
   void release() {
     for (DevBuf* b : {&merged, &flag, &excl, &scan, &gslot, &cnt, &ksum, &pat, &stats, &band4,
-                      &band4s, &cells, &cellcnt, &cellbase, &mpat, &cidx, &tw})
+                      &band4s, &cells, &cellcnt, &cellbase, &mpat, &cidx, &tw, &flag_b, &excl_b,
+                      &gslot_b, &sizes})
       b->release();
   }
 };
@@ -75,8 +77,10 @@ struct CIndex {
   DevBuf row_p, col, off, rows;
 };
 
-hipError_t build_strips(lamb_hip_ctx* ctx, SymScratch& W, const MatView& M, const DistView& D,
-                        bool a_side, StripSet& out);
+// The strips of A (a_side) and of B, built side by side so that their counts
+// and their packed panel sizes come back in one read each.
+hipError_t build_strips(lamb_hip_ctx* ctx, SymScratch& W, const MatView& MA, const MatView& MB,
+                        const DistView& D, StripSet& A, StripSet& B);
 // mask (optional): only blocks in the mask matrix's pattern are produced
 hipError_t build_tiles(lamb_hip_ctx* ctx, SymScratch& W, const DistView& D, const StripSet& A,
                        const StripSet& B, const MatView* Cold, double eps2, SymResult& r,

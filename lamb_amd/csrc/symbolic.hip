@@ -879,6 +879,16 @@ __global__ void xcd_bounds(int64_t nt, const int64_t* __restrict__ acc, int32_t*
   for (int x = xq + 1; x <= xp; ++x) xr[x] = (int32_t)p;
 }
 
+// out[0] = *a, out[1] = *b: two device counts gathered for one read
+__global__ void gather_pair(const int64_t* __restrict__ a, const int64_t* __restrict__ b,
+                            int64_t* __restrict__ out) {
+  LAMB_GUARD;
+  if (threadIdx.x == 0) {
+    out[0] = *a;
+    out[1] = *b;
+  }
+}
+
 // out[0..3] = the four chunk-total sums (after their scans, at [v*(nb+2)+nb]
 // with nb the number of scanned values), out[4..7] = kept, dropped, flops
 // and partial-entry statistics
@@ -901,13 +911,21 @@ __global__ void gather_totals(const int64_t* __restrict__ bts, int32_t nb,
 
 static unsigned grid_of(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
-hipError_t build_strips(lamb_hip_ctx* ctx, SymScratch& W, const MatView& M, const DistView& D,
-                        bool a_side, StripSet& out) {
-  hipStream_t s = ctx->stream;
+namespace {
+// scratch of one side of the strip build
+struct StripScratch {
+  DevBuf* flag;
+  DevBuf* excl;
+  DevBuf* gslot;
+};
+
+// slot flags and their scan: excl[nblk] = the side's strip count
+hipError_t strips_count(hipStream_t s, SymScratch& W, const StripScratch& sc, const MatView& M,
+                        const DistView& D, bool a_side) {
   const int64_t nblk = M.nblk;
   // the slot sequence: merged (A) or CSR (B) order
-  CK(W.flag.ensure(sizeof(int64_t) * (size_t)(nblk + 1)));
-  CK(W.excl.ensure(sizeof(int64_t) * (size_t)(nblk + 2)));
+  CK(sc.flag->ensure(sizeof(int64_t) * (size_t)(nblk + 1)));
+  CK(sc.excl->ensure(sizeof(int64_t) * (size_t)(nblk + 2)));
   CK(W.scan.ensure(sizeof(int64_t) * (size_t)scan_scratch_elems(nblk + 1)));
   if (a_side) {
     CK(W.merged.ensure(sizeof(int64_t) * (size_t)std::max<int64_t>(nblk, 1)));
@@ -918,11 +936,18 @@ hipError_t build_strips(lamb_hip_ctx* ctx, SymScratch& W, const MatView& M, cons
   if (nblk)
     hipLaunchKernelGGL(strip_flags, dim3(grid_of(nblk)), dim3(256), 0, s, nblk, a_side ? 1 : 0,
                        (const int64_t*)W.merged.p, M.rows, M.col, D.band_of,
-                       (int64_t*)W.flag.p);
+                       (int64_t*)sc.flag->p);
   CK(hipGetLastError());
-  CK(exclusive_scan(s, (const int64_t*)W.flag.p, (int64_t*)W.excl.p, nblk, (int64_t*)W.scan.p));
-  int64_t nstrips = 0;
-  CK(read_sizes(ctx, (int64_t*)W.excl.p + nblk, &nstrips, 1));
+  return exclusive_scan(s, (const int64_t*)sc.flag->p, (int64_t*)sc.excl->p, nblk,
+                        (int64_t*)W.scan.p);
+}
+
+// strips, group pointers, heads and packed panel offsets: pk[nstrips] = the
+// side's packed panel elements
+hipError_t strips_fill_side(hipStream_t s, SymScratch& W, const StripScratch& sc,
+                            const MatView& M, const DistView& D, bool a_side, int64_t nstrips,
+                            StripSet& out) {
+  const int64_t nblk = M.nblk;
   out.n = nstrips;
   CK(out.strips.ensure(sizeof(Strip) * (size_t)std::max<int64_t>(nstrips, 1)));
   if (nstrips)
@@ -930,38 +955,64 @@ hipError_t build_strips(lamb_hip_ctx* ctx, SymScratch& W, const MatView& M, cons
                        (Strip*)out.strips.p);
   if (nblk)
     hipLaunchKernelGGL(strip_fill, dim3(grid_of(nblk)), dim3(256), 0, s, nblk, a_side ? 1 : 0,
-                       (const int64_t*)W.merged.p, (const int64_t*)W.excl.p,
-                       (const int64_t*)W.flag.p, M.rows, M.col, M.off, M.nsq, D.band_of,
+                       (const int64_t*)W.merged.p, (const int64_t*)sc.excl->p,
+                       (const int64_t*)sc.flag->p, M.rows, M.col, M.off, M.nsq, D.band_of,
                        D.pos_in_band, (Strip*)out.strips.p);
   CK(hipGetLastError());
   // group pointers: per band (A) or per row atom (B)
   const int64_t ngroups = a_side ? D.nbands : D.n;
   CK(out.ptr.ensure(sizeof(int64_t) * (size_t)(ngroups + 1)));
-  CK(W.gslot.ensure(sizeof(int64_t) * (size_t)std::max<int64_t>(ngroups, 1)));
+  CK(sc.gslot->ensure(sizeof(int64_t) * (size_t)std::max<int64_t>(ngroups, 1)));
   if (a_side) {
     if (D.nbands)
       hipLaunchKernelGGL(band_first_slot, dim3(grid_of(D.nbands)), dim3(256), 0, s, D.nbands,
-                         D.bands, M.row_p, (int64_t*)W.gslot.p);
+                         D.bands, M.row_p, (int64_t*)sc.gslot->p);
   } else {
     if (D.n)
-      CK(hipMemcpyAsync(W.gslot.p, M.row_p, sizeof(int64_t) * (size_t)D.n,
+      CK(hipMemcpyAsync(sc.gslot->p, M.row_p, sizeof(int64_t) * (size_t)D.n,
                         hipMemcpyDeviceToDevice, s));
   }
   hipLaunchKernelGGL(group_ptr, dim3(grid_of(ngroups + 1)), dim3(256), 0, s, ngroups,
-                     (const int64_t*)W.gslot.p, (const int64_t*)W.excl.p, nblk, nstrips,
+                     (const int64_t*)sc.gslot->p, (const int64_t*)sc.excl->p, nblk, nstrips,
                      (int64_t*)out.ptr.p);
   CK(hipGetLastError());
-  // packed panel offsets
+  // packed panel offsets (the flags are consumed: their buffer takes the
+  // per-strip panel sizes)
   CK(out.pk.ensure(sizeof(int64_t) * (size_t)(nstrips + 1)));
-  CK(W.flag.ensure(sizeof(int64_t) * (size_t)(std::max(nblk, nstrips) + 1)));
+  CK(sc.flag->ensure(sizeof(int64_t) * (size_t)(std::max(nblk, nstrips) + 1)));
   CK(W.scan.ensure(sizeof(int64_t) * (size_t)scan_scratch_elems(std::max(nblk, nstrips) + 1)));
   CK(out.head.ensure(sizeof(StripHead) * (size_t)std::max<int64_t>(nstrips, 1)));
   if (nstrips)
     hipLaunchKernelGGL(strip_panel_size, dim3(grid_of(nstrips)), dim3(256), 0, s, nstrips,
-                       (const Strip*)out.strips.p, D.blk, D.bands, (int64_t*)W.flag.p,
+                       (const Strip*)out.strips.p, D.blk, D.bands, (int64_t*)sc.flag->p,
                        (StripHead*)out.head.p);
-  CK(exclusive_scan(s, (const int64_t*)W.flag.p, (int64_t*)out.pk.p, nstrips, (int64_t*)W.scan.p));
-  CK(read_sizes(ctx, (int64_t*)out.pk.p + nstrips, &out.pk_total, 1));
+  return exclusive_scan(s, (const int64_t*)sc.flag->p, (int64_t*)out.pk.p, nstrips,
+                        (int64_t*)W.scan.p);
+}
+}  // namespace
+
+hipError_t build_strips(lamb_hip_ctx* ctx, SymScratch& W, const MatView& MA, const MatView& MB,
+                        const DistView& D, StripSet& A, StripSet& B) {
+  hipStream_t s = ctx->stream;
+  const StripScratch sa{&W.flag, &W.excl, &W.gslot}, sb{&W.flag_b, &W.excl_b, &W.gslot_b};
+  CK(W.sizes.ensure(sizeof(int64_t) * 2));
+  int64_t* sizes = (int64_t*)W.sizes.p;
+  CK(strips_count(s, W, sa, MA, D, true));
+  CK(strips_count(s, W, sb, MB, D, false));
+  int64_t n[2] = {0, 0};  // strips of A, of B
+  hipLaunchKernelGGL(gather_pair, dim3(1), dim3(64), 0, s, (const int64_t*)W.excl.p + MA.nblk,
+                     (const int64_t*)W.excl_b.p + MB.nblk, sizes);
+  CK(hipGetLastError());
+  CK(read_sizes(ctx, sizes, n, 2));
+  CK(strips_fill_side(s, W, sa, MA, D, true, n[0], A));
+  CK(strips_fill_side(s, W, sb, MB, D, false, n[1], B));
+  int64_t pk[2] = {0, 0};  // packed panel elements of A, of B
+  hipLaunchKernelGGL(gather_pair, dim3(1), dim3(64), 0, s, (const int64_t*)A.pk.p + n[0],
+                     (const int64_t*)B.pk.p + n[1], sizes);
+  CK(hipGetLastError());
+  CK(read_sizes(ctx, sizes, pk, 2));
+  A.pk_total = pk[0];
+  B.pk_total = pk[1];
   return hipSuccess;
 }
 
