@@ -217,15 +217,15 @@ int32_t lamb_hip_trace_noalloc(const lamb_hip_mat* A, double* dev_trace);
  * a product's bits do not depend on timings; LAMB_HIP_F64_KERNEL=
  * quadrant|half|ksplit forces one (the variants add K products in different
  * orders: results then differ at rounding level). n_classes: workload classes
- * tuned so far. The autotuner times the tile orders (row bands per block of
- * the order in which workgroups take tiles: 4, 8, 16), each in three launch
- * forms (one workgroup per tile; a persistent grid of the device's resident
- * workgroups, each looping over tiles, the XCDs sharing one window of the
- * order; a persistent grid whose XCDs each walk their own work-balanced range
- * of the order), on the first product of a class and keeps the fastest (the
- * XCD ranges form, which moves fewer bytes, wins ties within 0.5 %); order
- * and launch form only map tiles to workgroups, so every candidate writes the
- * same bits. LAMB_HIP_TILE_ROWBLOCK=n and LAMB_HIP_PERSISTENT=0|1|2 force
+ * tuned so far. On the first product of a class the autotuner times the
+ * tile orders (row bands per block of the order in which workgroups take
+ * tiles: 4, 8, 16) as a persistent grid of the device's resident workgroups,
+ * each looping over tiles with the XCDs sharing one window of the order;
+ * then the fastest order as one workgroup per tile and as a persistent grid
+ * whose XCDs each walk their own work-balanced range of the order. It keeps
+ * the fastest of the five (the XCD ranges form, which moves fewer bytes,
+ * wins ties within 0.5 %); order and launch form only map tiles to
+ * workgroups, so every candidate writes the same bits. LAMB_HIP_TILE_ROWBLOCK=n and LAMB_HIP_PERSISTENT=0|1|2 force
  * them. */
 int32_t lamb_hip_tuning_info(const lamb_hip_ctx* ctx, int32_t* last_variant,
                              int32_t* n_classes);

@@ -285,7 +285,7 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
   // shape) comes from the rule unless shape tuning is on or a loaded table
   // names it: variants add K products in different orders, so only an
   // explicit request lets timings choose it. The autotuner then times the
-  // candidate tile orders, each in the three launch forms (one workgroup per
+  // candidate tile orders and the three launch forms (one workgroup per
   // tile, a persistent grid over the interleaved order, a persistent grid
   // with one range of the order per XCD), on the first product of a workload
   // class (form,
@@ -408,29 +408,40 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
   if (tune) {
     // every candidate writes the same C pool and kept masks; the last
     // launch's result is the call's, its time the call's numeric time
+    // Two stages, five launches: the tile orders in the interleaved
+    // persistent form, then the fastest order in the other two forms (the
+    // orders rank alike in every form). Within timing noise the XCD ranges
+    // form wins: it fetches a fifth to a quarter fewer bytes from beyond the
+    // L2 than the interleaved forms, which leaves the fabric to concurrent
+    // work, so the interleaved forms' times are compared 0.5 % higher.
     float best = 0.f;
     lamb_hip_ctx::TuneChoice best_choice = choice;
     int32_t best_grid = 0;
     bool first = true;
-    for (size_t c = 0; c < sizeof(kRowBlocks) / sizeof(kRowBlocks[0]); ++c) {
-      HIPCHK(ctx, build_tile_order(ctx, S.W, D, S.r, kRowBlocks[c]), "tile order");
-      for (int form = kFormGrid; form <= kFormXcdRanges; ++form) {
-        float ms = 0.f;
-        HIPCHK(ctx, timed(form, &ms), "numeric kernel (tuning)");
-        // Within timing noise the XCD ranges form wins: it fetches a fifth
-        // to a quarter fewer bytes from beyond the L2 than the interleaved
-        // forms, which leaves the fabric to concurrent work. The interleaved
-        // forms' times are compared 0.5 % higher.
-        if (form != kFormXcdRanges) ms *= 1.005f;
-        if (first || ms < best) {
-          best = ms;
-          best_choice.rowblock = kRowBlocks[c];
-          best_choice.form = form;
-          best_grid = grid;
-        }
-        first = false;
+    auto consider = [&](int32_t rowblock, int form) -> hipError_t {
+      float ms = 0.f;
+      const hipError_t e = timed(form, &ms);
+      if (e != hipSuccess) return e;
+      if (form != kFormXcdRanges) ms *= 1.005f;
+      if (first || ms < best) {
+        best = ms;
+        best_choice.rowblock = rowblock;
+        best_choice.form = form;
+        best_grid = grid;
       }
+      first = false;
+      return hipSuccess;
+    };
+    constexpr int kOrders = (int)(sizeof(kRowBlocks) / sizeof(kRowBlocks[0]));
+    for (int c = 0; c < kOrders; ++c) {
+      HIPCHK(ctx, build_tile_order(ctx, S.W, D, S.r, kRowBlocks[c]), "tile order");
+      HIPCHK(ctx, consider(kRowBlocks[c], kFormPersistent), "numeric kernel (tuning)");
     }
+    const int32_t order = best_choice.rowblock;
+    if (order != kRowBlocks[kOrders - 1])
+      HIPCHK(ctx, build_tile_order(ctx, S.W, D, S.r, order), "tile order");
+    HIPCHK(ctx, consider(order, kFormGrid), "numeric kernel (tuning)");
+    HIPCHK(ctx, consider(order, kFormXcdRanges), "numeric kernel (tuning)");
     ctx->tune[key] = best_choice;
     choice = best_choice;
     grid = best_grid;
