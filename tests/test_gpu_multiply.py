@@ -18,7 +18,7 @@ TOL = 1e-12
 
 
 def gpu_multiply(ctx, atoms, a, b, *, eps=0.0, alpha=1.0, beta=0.0, c_old=None, spatial=True,
-                 n_procs=1):
+                 n_procs=1, flags=None):
     dist = Distribution(atoms.blk, atoms.pos, atoms.box, n_procs=n_procs)
     dd = D.DeviceDistribution(ctx, dist, spatial=spatial)
     da = D.DeviceMatrix.from_lamb(dd, LambMatrix.from_global(dist, a))
@@ -26,7 +26,8 @@ def gpu_multiply(ctx, atoms, a, b, *, eps=0.0, alpha=1.0, beta=0.0, c_old=None, 
     dc = D.DeviceMatrix(dd)
     if c_old is not None:
         dc.upload(LambMatrix.from_global(dist, c_old))
-    st = D.multiply(da, db, dc, alpha=alpha, beta=beta, eps_filter=eps)
+    st = D.multiply(da, db, dc, alpha=alpha, beta=beta, eps_filter=eps,
+                    flags=D.FP64 if flags is None else flags)
     got = dc.download().to_global()
     for h in (dc, db, da, dd):
         h.close()
@@ -402,3 +403,26 @@ def test_block_norm_order_near_threshold(ctx, size):
     want = set(zip(rows[keep].tolist(), x.col[keep].tolist()))
     grows = np.repeat(np.arange(got.n_atoms), np.diff(got.row_p))
     assert set(zip(grows.tolist(), got.col.tolist())) == want
+
+
+@pytest.mark.parametrize("sizes", [(64,), (33, 31), (47, 17), (63, 2, 1)])
+@pytest.mark.parametrize("variant", ["quadrant", "half", "ksplit", "bf16"])
+def test_blocks_at_the_band_limit(ctx, monkeypatch, variant, sizes):
+    """Atoms of up to 64 rows, the band height: bands of one 64-row atom, of
+    two atoms that fill 64 rows exactly, and of many 1- and 2-row atoms (16
+    atoms, the band's atom limit). A dense product (every pair within the
+    cutoff) with a filter that binds near the cutoff, in each FP64 variant;
+    the BF16 split without a filter, within 1e-9."""
+    atoms, a = S.small_system(24, 90 + sizes[0], sizes=sizes, cutoff=7.0, decay=1.5)
+    b = S.decay_matrix(atoms, 7.0, 1.5, 91 + sizes[0])
+    if variant == "bf16":
+        print(f"case: bf16 sizes={sizes} atoms={atoms.n_atoms} blocks={a.n_blocks}", flush=True)
+        got, st = gpu_multiply(ctx, atoms, a, b, flags=D.BF16_SPLIT)
+        ref, rst = oracle.multiply(a, b)
+        same, err = compare(got, ref)
+        assert same and err <= 1e-9, err
+        assert st["n_triples_kept"] == rst["n_triples_kept"]
+        return
+    monkeypatch.setenv("LAMB_HIP_F64_KERNEL", variant)
+    print(f"variant {variant}, sizes {sizes}", flush=True)
+    check(ctx, atoms, a, b, eps=2e-3)
