@@ -145,6 +145,10 @@ constexpr int kSlotControlWords = 32;
 constexpr int kF64Quadrant = 0;
 constexpr int kF64Ksplit = 1;
 constexpr int kF64Half = 2;
+constexpr int kF64Lanes = 3;  // vector-ALU FMAs, K columns over lane groups (whole products)
+// Panels addressable by 32-bit byte offsets (a_zero, b_zero: the zero column
+// after each side's panels)
+bool lanes_panels_fit(uint32_t a_zero, uint32_t b_zero);
 hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* tiles,
                               const int32_t* perm, const Entry* entries, const Band* bands,
                               const double* a_pack, const double* b_pack, uint32_t a_zero,

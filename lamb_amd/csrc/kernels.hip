@@ -185,14 +185,14 @@ constexpr int kGroups = kStepCols / 4;
 // window is partial (its atoms cut below its strips'). Returns the window's
 // step count; the table and *partial are visible to all threads on return.
 // tid = threadIdx.x.
-template <int KS>
+template <int KS, int NCOLS = kTableCols>
 __device__ __forceinline__ int build_step_table(const TileInfo& T, const TileSmem& sm,
                                                 const Entry* __restrict__ entries, int Q0,
                                                 uint2* col_ab, uint32_t* col_m, uint32_t* step_g,
                                                 uint32_t a_zero, uint32_t b_zero, int* partial,
                                                 int tid) {
   static_assert(KS == 4 || KS == 8, "K columns per step");
-  const int ncols = min(kTableCols, T.ktot - Q0);
+  const int ncols = min(NCOLS, T.ktot - Q0);
   const int nsteps = (ncols + KS - 1) / KS;
   __syncthreads();  // the previous window is done with the table
   if (tid == 0) *partial = 0;
@@ -617,7 +617,7 @@ __global__ __launch_bounds__(256, 2) void numeric_f64_ksplit_kernel(
 #endif
 constexpr int kHalfPrefetch = LAMB_HALF_PREFETCH;
 
-template <bool kWhole, bool kLoop>
+template <bool kWhole, bool kLoop, bool kRuns = false>
 __global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
     int32_t n_tiles, int32_t* __restrict__ slot_counter, const int32_t* __restrict__ xrange,
     const TileDesc* __restrict__ tiles,
@@ -672,6 +672,12 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
     const int kc = lane >> 4;
     const double* a_lane = a_pack + (lane & 15);
     const double* b_lane = b_pack + 32 * wc + (lane & 15);
+    // byte offsets of this lane's value in a panel column and in a run of
+    // four columns (kRuns)
+    const char* a_bytes = reinterpret_cast<const char*>(a_pack);
+    const char* b_bytes = reinterpret_cast<const char*>(b_pack);
+    const uint32_t row_a = 8u * (lane & 15), row_b = 8u * (32 * wc + (lane & 15));
+    const uint32_t run_a = row_a + 512u * kc, run_b = row_b + 512u * kc;
     // bits 4h + 2wc + q of a tile block mask: this wave's half
     const uint32_t half_bits = 0x3333u << (2 * wc);
 
@@ -686,6 +692,12 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
       int cbase = -64;
       uint64_t live = 0;
       uint32_t chunk_bm = 0;
+      // with kRuns, lane l of a chunk also holds step cbase + l's first A and
+      // B panel columns, and bit 16 of its block mask says whether the step's
+      // 4 columns are consecutive in both panels (inside one entry they are):
+      // such a step loads from a wave-uniform column and fixed lane offsets,
+      // with no table read between finding the step and loading it
+      uint32_t chunk_a = 0, chunk_b = 0;
       double ra[8][4], rb[8][2];
       uint32_t blkmask[8];
   #define HF_FETCH(S)                                                                \
@@ -696,6 +708,22 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
           cbase += 64;                                                               \
           const int sl = cbase + lane;                                               \
           chunk_bm = tile_blocks(sl < nsteps ? step_g[sl] : 0u) & half_bits;         \
+          if constexpr (kRuns) {                                                     \
+            uint32_t c0a = 0, c0b = 0;                                               \
+            if (chunk_bm) {                                                          \
+              const uint2 c0 = col_ab[4 * sl];                                       \
+              c0a = c0.x;                                                            \
+              c0b = c0.y;                                                            \
+              bool run = true;                                                       \
+              _Pragma("unroll") for (int q = 1; q < 4; ++q) {                        \
+                const uint2 c = col_ab[4 * sl + q];                                  \
+                run = run && c.x == c0a + q && c.y == c0b + q;                       \
+              }                                                                      \
+              if (run) chunk_bm |= 1u << 16;                                         \
+            }                                                                        \
+            chunk_a = c0a;                                                           \
+            chunk_b = c0b;                                                           \
+          }                                                                          \
           /* this wave's live steps: those of rank parity par in the half */         \
           const uint64_t any = __ballot(chunk_bm != 0u);                             \
           const uint32_t rank = seen + __builtin_amdgcn_mbcnt_hi(                    \
@@ -703,13 +731,35 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
           live = __ballot(chunk_bm != 0u && (rank & 1u) == par);                     \
           seen += (uint32_t)__builtin_popcountll(any);                               \
         }                                                                            \
+        uint32_t a0 = 0, b0 = 0;                                                     \
         if (live) {                                                                  \
           const int l = __builtin_ctzll(live);                                       \
           live &= live - 1;                                                          \
           st = cbase + l;                                                            \
           bm = __builtin_amdgcn_readlane(chunk_bm, l);                               \
+          if constexpr (kRuns) {                                                     \
+            a0 = __builtin_amdgcn_readlane(chunk_a, l);                              \
+            b0 = __builtin_amdgcn_readlane(chunk_b, l);                              \
+          }                                                                          \
         }                                                                            \
-        blkmask[S] = bm;                                                             \
+        blkmask[S] = bm & 0xFFFFu;                                                   \
+        if constexpr (kRuns) {                                                       \
+          /* byte offsets (panels below 4 GB); without a step: step 0 */             \
+          uint32_t va, vb;                                                           \
+          if (bm & (1u << 16)) {                                                     \
+            va = a0 * 512u + run_a;                                                  \
+            vb = b0 * 512u + run_b;                                                  \
+          } else {                                                                   \
+            const uint2 ab = col_ab[4 * st + kc];                                    \
+            va = ab.x * 512u + row_a;                                                \
+            vb = ab.y * 512u + row_b;                                                \
+          }                                                                          \
+          _Pragma("unroll") for (int h = 0; h < 4; ++h)                              \
+            ra[S][h] = *reinterpret_cast<const double*>(a_bytes + va + 128 * h);     \
+          _Pragma("unroll") for (int q = 0; q < 2; ++q)                              \
+            rb[S][q] = *reinterpret_cast<const double*>(b_bytes + vb + 128 * q);     \
+          __builtin_amdgcn_sched_barrier(0);                                         \
+        } else {                                                                     \
         const int Q = 4 * st + kc;                                                   \
         const uint2 ab = col_ab[Q];                                                  \
         if constexpr (kWhole) {                                                      \
@@ -731,6 +781,7 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
             const uint32_t c = (cm & need) ? ab.y : b_zero;                          \
             rb[S][q] = b_lane[(int64_t)c * 64 + 16 * q];                             \
           }                                                                          \
+        }                                                                            \
         }                                                                            \
       }
   #define HF_MFMA(S)                                                                 \
@@ -794,6 +845,257 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// FP64 numeric phase on the vector ALU, for whole products. One workgroup (4
+// waves) per C tile; wave g owns the 16-row group g of the tile and walks the
+// tile twice, once per 32-column half. Lane (kc, r) = (lane >> 4, lane & 15)
+// holds row 16g + r and accumulates, for the 32 columns of the half, the K
+// columns of each step of 4 whose position in the step is kc. Per step the
+// wave loads the A panel column values of its 16 rows (one per lane, the
+// v_mfma_f64_16x16x4 A layout) and the B values of the half's 32 columns
+// (two loads, the MFMA B layout), then runs up to 32 v_fmac_f64 whose B
+// operand is broadcast from lane (kc, j) to the lane group kc by DPP
+// row_newbcast:j. v_fmac_f64 issues at the full FP64 rate, against about
+// 100 cycles per v_mfma_f64_16x16x4, and the live-block granularity is 16
+// rows x 4 columns instead of 16 x 16: a step skips a wave whose row group
+// is dead, and a 4-column group whose columns are dead. After a half's walk
+// the four K partials of each value are added across the lane groups (xor
+// 32, then xor 16: the same two-level sum in every lane), and lane (kc, r)
+// keeps the columns 4j + kc. Panels are zero outside their strips' atoms and
+// every entry is whole, so dead rows and columns inside a live group add
+// exact zeros.
+// ---------------------------------------------------------------------------
+// K columns of the lanes kernel's step table, and where its first half
+// waits (in doubles from the start of the tile's LDS buffer)
+constexpr int kLanesCols = 1024;
+constexpr int kLanesHalf0 = 1664;
+#ifndef LAMB_LANES_DEPTH
+#define LAMB_LANES_DEPTH 2
+#endif
+constexpr int kLanesDepth = LAMB_LANES_DEPTH;  // steps in flight per wave (1..3)
+static_assert(kLanesDepth >= 1 && kLanesDepth <= 3, "lanes kernel depth");
+
+#define LAMB_FMAC_BCAST(ACC, B, A, J)                                                        \
+  asm("v_fmac_f64_dpp %0, %1, %2 row_newbcast:" #J " row_mask:0xf bank_mask:0xf"          \
+      : "+v"(ACC)                                                                          \
+      : "v"(B), "v"(A))
+
+template <bool kLoop>
+__global__ __launch_bounds__(256, 4) void numeric_f64_lanes_kernel(
+    int32_t n_tiles, int32_t* __restrict__ slot_counter, const int32_t* __restrict__ xrange,
+    const TileDesc* __restrict__ tiles,
+    const int32_t* __restrict__ perm,
+    const Entry* __restrict__ entries, const Band* __restrict__ bands,
+    const double* __restrict__ a_pack, const double* __restrict__ b_pack, uint32_t a_zero,
+    uint32_t b_zero, double alpha, double beta, double eps2,
+    const int64_t* __restrict__ old_row_p, const int32_t* __restrict__ old_col,
+    const int64_t* __restrict__ old_off, const double* __restrict__ old_data,
+    double* __restrict__ c_data, uint16_t* __restrict__ kept_out) {
+  LAMB_GUARD;
+  __shared__ int32_t next_slot;
+  int32_t slot = (int32_t)blockIdx.x;
+  if constexpr (kLoop) {
+    slot = persistent_slot(xrange, n_tiles, (int32_t)(blockIdx.x >> 3));
+    if (slot < 0) return;
+  }
+  for (;;) {
+    // main loop: the step table of a window of kLanesCols K columns, then
+    // the reduced first half (64 x 32 values); epilogue: C tile
+    __shared__ double lds[kBandRows * kPitch];
+    uint2* col_ab = reinterpret_cast<uint2*>(lds);
+    uint32_t* col_m = reinterpret_cast<uint32_t*>(col_ab + kLanesCols);
+    uint32_t* step_g = col_m + kLanesCols;
+    double* half0 = lds + kLanesHalf0;
+    static_assert(sizeof(uint32_t) * (3 * kLanesCols + kLanesCols / 4) <=
+                      sizeof(double) * kLanesHalf0 &&
+                  kLanesHalf0 + kBandRows * 32 <= kBandRows * kPitch,
+                  "step table and first half exceed the LDS buffer");
+    __shared__ TileSmem sm;
+    __shared__ int win_partial;
+
+    const int tid = tile_tid<kLoop>();
+    const int lane = tid & 63;
+    const int g = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int kc = lane >> 4, r = lane & 15;
+    const TileInfo T = tile_setup(tiles, perm, bands, sm, slot, tid);
+    const char* a_bytes = reinterpret_cast<const char*>(a_pack);
+    const char* b_bytes = reinterpret_cast<const char*>(b_pack);
+    const uint32_t row_a = (uint32_t)(8 * (16 * g + r));  // byte offset in a panel column
+    const uint32_t off_a = row_a + 512u * kc;             // ... and in a run of columns
+    // lane (kc, r): C(16g + r, 32 + 4j + kc) after the second half
+    double keep1[8];
+
+#pragma unroll 1
+    for (int h = 0; h < 2; ++h) {
+      const uint32_t row_b = (uint32_t)(8 * (32 * h + r));
+      const uint32_t off_b = row_b + 512u * kc;
+      double acc[32];
+#pragma unroll
+      for (int x = 0; x < 32; ++x) acc[x] = 0.0;
+      for (int Q0 = 0; Q0 < T.ktot; Q0 += kLanesCols) {
+        // one window's table serves both halves
+        const int nsteps =
+            (h == 0 || T.ktot > kLanesCols)
+                ? build_step_table<8, kLanesCols>(T, sm, entries, Q0, col_ab, col_m, step_g,
+                                                  a_zero, b_zero, &win_partial, tid)
+                : (min(kLanesCols, T.ktot - Q0) + 7) / 8;
+        // Steps of 8 K columns: lane (kc, r) takes columns kc and 4 + kc. Per
+        // chunk of 64 steps, lane l reads step cbase + l's first panel
+        // columns and whether its 8 columns are consecutive in both panels
+        // (bit 8 of the mask; inside one entry they are): such a step's loads
+        // take a wave-uniform base and fixed lane offsets, with no table read
+        // between finding the step and loading its operands.
+        int cbase = -64;
+        uint64_t live = 0;
+        uint32_t chunk_m = 0, chunk_a = 0, chunk_b = 0;
+        double ra[3][2], rb[3][4];
+        uint32_t cm[3];
+#define LN_FETCH(S)                                                                   \
+        {                                                                             \
+          while (live == 0 && cbase + 64 < nsteps) {                                  \
+            cbase += 64;                                                              \
+            const int sl = cbase + lane;                                              \
+            uint32_t m = 0, c0a = 0, c0b = 0;                                         \
+            if (sl < nsteps) {                                                        \
+              const uint32_t sg = step_g[sl];                                         \
+              m = ((sg >> (4 * g)) & 0xFu) ? ((sg >> (16 + 8 * h)) & 0xFFu) : 0u;     \
+              if (m) {                                                                \
+                const uint2 c0 = col_ab[8 * sl];                                      \
+                c0a = c0.x;                                                           \
+                c0b = c0.y;                                                           \
+                bool run = true;                                                      \
+                _Pragma("unroll") for (int q = 1; q < 8; ++q) {                       \
+                  const uint2 c = col_ab[8 * sl + q];                                 \
+                  run = run && c.x == c0a + q && c.y == c0b + q;                      \
+                }                                                                     \
+                if (run) m |= 0x100u;                                                 \
+              }                                                                       \
+            }                                                                         \
+            chunk_m = m;                                                              \
+            chunk_a = c0a;                                                            \
+            chunk_b = c0b;                                                            \
+            live = __ballot(m != 0u);                                                 \
+          }                                                                           \
+          uint32_t m = 0, a0 = 0, b0 = 0;                                             \
+          int st = 0;                                                                 \
+          if (live) {                                                                 \
+            const int l = __builtin_ctzll(live);                                      \
+            live &= live - 1;                                                         \
+            st = cbase + l;                                                           \
+            m = __builtin_amdgcn_readlane(chunk_m, l);                                \
+            a0 = __builtin_amdgcn_readlane(chunk_a, l);                               \
+            b0 = __builtin_amdgcn_readlane(chunk_b, l);                               \
+          }                                                                           \
+          cm[S] = m & 0xFFu;                                                          \
+          /* byte offsets into the panels (below 4 GB each, which the variant  */   \
+          /* rule checks): a run of 8 columns from the step's first columns,    */   \
+          /* else each lane's own columns from the table                        */   \
+          uint32_t va0, va1, vb0, vb1;                                                \
+          if (m & 0x100u) {                                                           \
+            va0 = a0 * 512u + off_a;                                                  \
+            va1 = va0 + 2048u;                                                        \
+            vb0 = b0 * 512u + off_b;                                                  \
+            vb1 = vb0 + 2048u;                                                        \
+          } else {                                                                    \
+            /* without a step the loads read step 0 and feed no FMA */               \
+            const uint2 c0 = col_ab[8 * st + kc], c1 = col_ab[8 * st + 4 + kc];       \
+            va0 = c0.x * 512u + row_a;                                                \
+            va1 = c1.x * 512u + row_a;                                                \
+            vb0 = c0.y * 512u + row_b;                                                \
+            vb1 = c1.y * 512u + row_b;                                                \
+          }                                                                           \
+          ra[S][0] = *reinterpret_cast<const double*>(a_bytes + va0);                 \
+          ra[S][1] = *reinterpret_cast<const double*>(a_bytes + va1);                 \
+          rb[S][0] = *reinterpret_cast<const double*>(b_bytes + vb0);                 \
+          rb[S][1] = *reinterpret_cast<const double*>(b_bytes + vb0 + 128);           \
+          rb[S][2] = *reinterpret_cast<const double*>(b_bytes + vb1);                 \
+          rb[S][3] = *reinterpret_cast<const double*>(b_bytes + vb1 + 128);           \
+          __builtin_amdgcn_sched_barrier(0);                                          \
+        }
+#define LN_GROUP(S, C, X, J0, J1, J2, J3)                                             \
+        if (__builtin_expect((cm[S] >> (X)) & 1u, 1)) {                               \
+          LAMB_FMAC_BCAST(acc[4 * (X) + 0], rb[S][C], ra[S][0], J0);                  \
+          LAMB_FMAC_BCAST(acc[4 * (X) + 1], rb[S][C], ra[S][0], J1);                  \
+          LAMB_FMAC_BCAST(acc[4 * (X) + 2], rb[S][C], ra[S][0], J2);                  \
+          LAMB_FMAC_BCAST(acc[4 * (X) + 3], rb[S][C], ra[S][0], J3);                  \
+          LAMB_FMAC_BCAST(acc[4 * (X) + 0], rb[S][C + 2], ra[S][1], J0);              \
+          LAMB_FMAC_BCAST(acc[4 * (X) + 1], rb[S][C + 2], ra[S][1], J1);              \
+          LAMB_FMAC_BCAST(acc[4 * (X) + 2], rb[S][C + 2], ra[S][1], J2);              \
+          LAMB_FMAC_BCAST(acc[4 * (X) + 3], rb[S][C + 2], ra[S][1], J3);              \
+        }
+#define LN_FMA(S)                                                                     \
+        {                                                                             \
+          LN_GROUP(S, 0, 0, 0, 1, 2, 3)                                               \
+          LN_GROUP(S, 0, 1, 4, 5, 6, 7)                                               \
+          LN_GROUP(S, 0, 2, 8, 9, 10, 11)                                             \
+          LN_GROUP(S, 0, 3, 12, 13, 14, 15)                                           \
+          LN_GROUP(S, 1, 4, 0, 1, 2, 3)                                               \
+          LN_GROUP(S, 1, 5, 4, 5, 6, 7)                                               \
+          LN_GROUP(S, 1, 6, 8, 9, 10, 11)                                             \
+          LN_GROUP(S, 1, 7, 12, 13, 14, 15)                                           \
+        }
+#define LN_SLOTS(X) X(0) X(1) X(2)
+#define LN_PRIME(S) if (S < kLanesDepth) LN_FETCH(S)
+#define LN_ANY(S) if (S < kLanesDepth) any |= cm[S];
+#define LN_STEP(S) if (S < kLanesDepth) { LN_FMA(S); LN_FETCH(S); }
+        LN_SLOTS(LN_PRIME)
+        for (;;) {
+          uint32_t any = 0;
+          LN_SLOTS(LN_ANY)
+          if (!any) break;
+          LN_SLOTS(LN_STEP)
+        }
+#undef LN_STEP
+#undef LN_ANY
+#undef LN_PRIME
+#undef LN_SLOTS
+#undef LN_FMA
+#undef LN_GROUP
+#undef LN_FETCH
+      }
+      // the four K partials of column c = 4j + kc, summed into lane group kc
+      // as (p_kc + p_kc^2) + (p_kc^1 + p_kc^3): across xor 32 each lane group
+      // keeps the column pair whose bit 1 is its own, across xor 16 the
+      // column of its own parity
+      const bool up = (kc & 2) != 0, odd = (kc & 1) != 0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        double k0 = up ? acc[4 * j + 2] : acc[4 * j];
+        double k1 = up ? acc[4 * j + 3] : acc[4 * j + 1];
+        const double s0 = up ? acc[4 * j] : acc[4 * j + 2];
+        const double s1 = up ? acc[4 * j + 1] : acc[4 * j + 3];
+        k0 += __shfl_xor(s0, 32);
+        k1 += __shfl_xor(s1, 32);
+        double v = odd ? k1 : k0;
+        v += __shfl_xor(odd ? k0 : k1, 16);
+        // the first half waits in LDS beside the table (rows of 32 values)
+        if (h == 0) half0[(16 * g + r) * 32 + 4 * j + kc] = v;
+        else keep1[j] = v;
+      }
+    }
+    double keep0[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) keep0[j] = half0[(16 * g + r) * 32 + 4 * j + kc];
+    __syncthreads();  // every wave is done with the table and the first half
+
+    double* Ct = lds;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      Ct[(16 * g + r) * kPitch + 4 * j + kc] = keep0[j];
+      Ct[(16 * g + r) * kPitch + 32 + 4 * j + kc] = keep1[j];
+    }
+    __syncthreads();
+    tile_epilogue<256>(Ct, T, sm, alpha, beta, eps2, old_row_p, old_col, old_off, old_data,
+                       c_data, kept_out, tid,
+                       kLoop ? slot_counter + (blockIdx.x & 7) : nullptr, &next_slot);
+    if constexpr (!kLoop) break;
+    __syncthreads();  // next_slot is visible; the next set-up rewrites the LDS
+    slot = persistent_slot(xrange, n_tiles, __builtin_amdgcn_readfirstlane(next_slot));
+    if (slot < 0) break;
+  }
+}
+#undef LAMB_FMAC_BCAST
+
 // Workgroups of a 256-thread kernel resident on the current device at once
 // (resident per CU times the CUs): the grid of a persistent launch. Cached
 // per kernel and device; 0 when the runtime cannot tell.
@@ -828,6 +1130,13 @@ __global__ void slot_counter_reset(int32_t* __restrict__ counter, int32_t grid) 
 
 void launch_slot_counter_reset(hipStream_t s, int32_t* counter, int32_t grid) {
   hipLaunchKernelGGL(slot_counter_reset, dim3(1), dim3(64), 0, s, counter, grid);
+}
+
+// The lanes kernel and the half kernel's run walk address the panels by
+// 32-bit byte offsets: both panels, the zero column after the last one
+// included, must stay below 4 GB.
+bool lanes_panels_fit(uint32_t a_zero, uint32_t b_zero) {
+  return ((uint64_t)std::max(a_zero, b_zero) + 1) * 512u <= 0xFFFFFFFFull;
 }
 
 hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* tiles,
@@ -866,10 +1175,16 @@ hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* ti
   else if (variant == kF64Ksplit)
     form != kFormGrid ? go(numeric_f64_ksplit_kernel<false, true>)
                : go(numeric_f64_ksplit_kernel<false, false>);
+  else if (variant == kF64Lanes && whole && lanes_panels_fit(a_zero, b_zero))
+    form != kFormGrid ? go(numeric_f64_lanes_kernel<true>)
+               : go(numeric_f64_lanes_kernel<false>);
+  else if (variant == kF64Half && whole && lanes_panels_fit(a_zero, b_zero))
+    form != kFormGrid ? go(numeric_f64_half_kernel<true, true, true>)
+               : go(numeric_f64_half_kernel<true, false, true>);
   else if (variant == kF64Half && whole)
     form != kFormGrid ? go(numeric_f64_half_kernel<true, true>)
                : go(numeric_f64_half_kernel<true, false>);
-  else if (variant == kF64Half)
+  else if (variant == kF64Half || variant == kF64Lanes)  // the lanes kernel: whole products
     form != kFormGrid ? go(numeric_f64_half_kernel<false, true>)
                : go(numeric_f64_half_kernel<false, false>);
   else

@@ -1,6 +1,6 @@
 """Tile-shape tuning and the persisted tuning table.
 
-With shape tuning on, the first product of a workload class times the three
+With shape tuning on, the first product of a workload class times the
 FP64 per-wave tile shapes and keeps the fastest for the class; the class key
 includes the distribution's set of block sizes. A table saved from one
 context and loaded into another reproduces every choice, so the second
