@@ -42,7 +42,8 @@ LAUNCH_FORMS = {0: "one workgroup per tile", 1: "persistent grid, XCDs share one
                 2: "persistent grid, one range of the tile order per XCD"}
 # FP64 numeric kernel of each variant (lamb_hip_tuning_info)
 F64_KERNELS = {0: "numeric_f64_kernel", 1: "numeric_f64_ksplit_kernel",
-               2: "numeric_f64_half_kernel", 3: "numeric_f64_lanes_kernel"}
+               2: "numeric_f64_half_kernel", 3: "numeric_f64_lanes_kernel",
+               4: "numeric_f64_half_kernel (fine)"}
 
 
 def parse():
@@ -132,7 +133,7 @@ def cpu_baseline(a, b, eps, every):
 def shape_choices(D, ctx):
     """{tile shape: classes} of the context's tuning table (shape entries)."""
     import tempfile
-    names = {0: "quadrant 32x32", 1: "ksplit 64x64", 2: "half 64x32", 3: "lanes 16x32"}
+    names = {0: "quadrant 32x32", 1: "ksplit 64x64", 2: "half 64x32", 3: "lanes 16x32", 4: "fine 64x32"}
     with tempfile.TemporaryDirectory() as tmp:
         path = os.path.join(tmp, "tuning.txt")
         D.tuning_save(ctx, path)

@@ -213,11 +213,13 @@ int32_t lamb_hip_trace_noalloc(const lamb_hip_mat* A, double* dev_trace);
  * (0 quadrant: per-wave 32 x 32 quadrants; 1 ksplit: whole-tile waves
  * splitting the K steps four ways; 2 half: per-wave 64 x 32 halves, two waves
  * splitting each half's K steps; 3 lanes: vector-ALU FMAs, per-wave 16 x 32
- * with the K columns of a step spread over the lane groups, for whole
- * products whose packed panels stay below 4 GB, else half). The variant is a
+ * with the K columns of a step spread over the lane groups; 4 fine: the half
+ * kernel on v_mfma_f64_4x4x4_4b, 16-row x 4-column live groups; lanes and
+ * fine serve whole products whose packed panels stay below 4 GB, others run
+ * half). The variant is a
  * function of the product (K-split for whole products of at least 1e7 flop
  * per tile, else half), so a product's bits do not depend on timings;
- * LAMB_HIP_F64_KERNEL=quadrant|half|ksplit|lanes forces one (the variants add
+ * LAMB_HIP_F64_KERNEL=quadrant|half|ksplit|lanes|fine forces one (the variants add
  * K products in different orders: results then differ at rounding level). n_classes: workload classes
  * tuned so far. On the first product of a class the autotuner times the
  * tile orders (row bands per block of the order in which workgroups take
@@ -236,12 +238,12 @@ int32_t lamb_hip_tuning_info(const lamb_hip_ctx* ctx, int32_t* last_variant,
 int32_t lamb_hip_tuning_order(const lamb_hip_ctx* ctx, int32_t* last_rowblock);
 /* Tile-shape tuning (no counterpart in the reference). The per-wave tile of
  * the FP64 kernel (quadrant 32 x 32, half 64 x 32, K-split 64 x 64, lanes
- * 16 x 32; see lamb_hip_tuning_info) is by default chosen by the rule, because the shapes
+ * 16 x 32, fine 64 x 32 on 4 x 4 x 4 MFMAs; see lamb_hip_tuning_info) is by default chosen by the rule, because the shapes
  * add K products in different orders. tune_shapes = 1: the first product of
  * each workload class (form, whether every entry is whole, the set of block
  * sizes of the distribution, i.e. the (m, n, k) shapes of its blocks, and
  * half-octave buckets of flops and of C blocks per tile) times the shapes
- * (lanes only for whole products) and keeps the fastest for that class; results then depend, at
+ * (lanes and fine only for whole products) and keeps the fastest for that class; results then depend, at
  * rounding level, on which shape won. tune_shapes = 0 (default): rule, or a
  * loaded table's entry. */
 int32_t lamb_hip_tuning_mode(lamb_hip_ctx* ctx, int32_t tune_shapes);

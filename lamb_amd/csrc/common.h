@@ -146,6 +146,7 @@ constexpr int kF64Quadrant = 0;
 constexpr int kF64Ksplit = 1;
 constexpr int kF64Half = 2;
 constexpr int kF64Lanes = 3;  // vector-ALU FMAs, K columns over lane groups (whole products)
+constexpr int kF64Fine = 4;   // half kernel on 4x4x4 MFMAs, 16-row x 4-column groups (whole)
 // Panels addressable by 32-bit byte offsets (a_zero, b_zero: the zero column
 // after each side's panels)
 bool lanes_panels_fit(uint32_t a_zero, uint32_t b_zero);

@@ -612,12 +612,52 @@ __global__ __launch_bounds__(256, 2) void numeric_f64_ksplit_kernel(
 // loads). The two partial halves are added in LDS: a two-term sum, so the
 // result does not depend on which wave writes first.
 // ---------------------------------------------------------------------------
+// The fine form's live groups of a step for the column half wc of a tile:
+// bits 0-3 the 16-row groups holding a live 4-row group, bits 4-11 the
+// half's live 4-column groups; 0 when either is empty.
+__device__ __forceinline__ uint32_t fine_groups(uint32_t g, int wc) {
+  const uint32_t rg = g & 0xFFFFu;
+  const uint32_t cg = (g >> (16 + 8 * wc)) & 0xFFu;
+  uint32_t rh = 0;
+#pragma unroll
+  for (int h = 0; h < 4; ++h) rh |= ((rg >> (4 * h)) & 0xFu) ? 1u << h : 0u;
+  return (rh && cg) ? (rh | (cg << 4)) : 0u;
+}
+
+// B operand of v_mfma_f64_4x4x4_4b with all four blocks on the columns
+// 4c..4c+3 of a 16-column block, from that block's v_mfma_f64_16x16x4 B
+// operand (lane 16k + j holds B[k][j]): lane 16k + 4b + i takes lane
+// 16k + 4c + i, a ds_swizzle in bit mode (and 0x13, or 4c) on each half.
+__device__ __forceinline__ double replicate_cols(double v, int c) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  uint32_t lo, hi;
+  switch (c) {  // the pattern is an immediate
+    case 0:
+      lo = (uint32_t)__builtin_amdgcn_ds_swizzle((int)(uint32_t)u, 0x13);
+      hi = (uint32_t)__builtin_amdgcn_ds_swizzle((int)(uint32_t)(u >> 32), 0x13);
+      break;
+    case 1:
+      lo = (uint32_t)__builtin_amdgcn_ds_swizzle((int)(uint32_t)u, 0x13 | (4 << 5));
+      hi = (uint32_t)__builtin_amdgcn_ds_swizzle((int)(uint32_t)(u >> 32), 0x13 | (4 << 5));
+      break;
+    case 2:
+      lo = (uint32_t)__builtin_amdgcn_ds_swizzle((int)(uint32_t)u, 0x13 | (8 << 5));
+      hi = (uint32_t)__builtin_amdgcn_ds_swizzle((int)(uint32_t)(u >> 32), 0x13 | (8 << 5));
+      break;
+    default:
+      lo = (uint32_t)__builtin_amdgcn_ds_swizzle((int)(uint32_t)u, 0x13 | (12 << 5));
+      hi = (uint32_t)__builtin_amdgcn_ds_swizzle((int)(uint32_t)(u >> 32), 0x13 | (12 << 5));
+      break;
+  }
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
 #ifndef LAMB_HALF_PREFETCH
 #define LAMB_HALF_PREFETCH 3
 #endif
 constexpr int kHalfPrefetch = LAMB_HALF_PREFETCH;
 
-template <bool kWhole, bool kLoop, bool kRuns = false>
+template <bool kWhole, bool kLoop, bool kRuns = false, bool kFine = false>
 __global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
     int32_t n_tiles, int32_t* __restrict__ slot_counter, const int32_t* __restrict__ xrange,
     const TileDesc* __restrict__ tiles,
@@ -681,9 +721,19 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
     // bits 4h + 2wc + q of a tile block mask: this wave's half
     const uint32_t half_bits = 0x3333u << (2 * wc);
 
+    static_assert(!kFine || (kWhole && kRuns), "the fine form serves whole products");
     dbl4 acc[8];
+    // kFine: row group h x 16-column block q x 4-column group c, one
+    // v_mfma_f64_4x4x4_4b accumulator each (16 rows x 4 columns)
+    double accf[4][2][4];
   #pragma unroll
     for (int x = 0; x < 8; ++x) acc[x] = dbl4{0.0, 0.0, 0.0, 0.0};
+  #pragma unroll
+    for (int h = 0; h < 4; ++h)
+  #pragma unroll
+      for (int q = 0; q < 2; ++q)
+  #pragma unroll
+        for (int c = 0; c < 4; ++c) accf[h][q][c] = 0.0;
     uint32_t seen = 0;  // live steps of this half met so far (wave-uniform)
 
     for (int Q0 = 0; Q0 < T.ktot; Q0 += kTableCols) {
@@ -707,7 +757,10 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
         while (live == 0 && cbase + 64 < nsteps) {                                   \
           cbase += 64;                                                               \
           const int sl = cbase + lane;                                               \
-          chunk_bm = tile_blocks(sl < nsteps ? step_g[sl] : 0u) & half_bits;         \
+          if constexpr (kFine)                                                       \
+            chunk_bm = fine_groups(sl < nsteps ? step_g[sl] : 0u, wc);               \
+          else                                                                       \
+            chunk_bm = tile_blocks(sl < nsteps ? step_g[sl] : 0u) & half_bits;       \
           if constexpr (kRuns) {                                                     \
             uint32_t c0a = 0, c0b = 0;                                               \
             if (chunk_bm) {                                                          \
@@ -785,7 +838,18 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
         }                                                                            \
       }
   #define HF_MFMA(S)                                                                 \
-      {                                                                              \
+      if constexpr (kFine) {                                                         \
+        /* bits 0-3: live 16-row groups; bits 4 + 4q + c: live 4-column groups */    \
+        const uint32_t bm = blkmask[S];                                              \
+        _Pragma("unroll") for (int q = 0; q < 2; ++q)                                \
+          _Pragma("unroll") for (int c = 0; c < 4; ++c)                              \
+            if (__builtin_expect((bm >> (4 + 4 * q + c)) & 1u, 1)) {                 \
+              const double bc = replicate_cols(rb[S][q], c);                         \
+              _Pragma("unroll") for (int h = 0; h < 4; ++h)                          \
+                if (__builtin_expect((bm >> h) & 1u, 1))                             \
+                  accf[h][q][c] = __builtin_amdgcn_mfma_f64_4x4x4f64(ra[S][h], bc, accf[h][q][c], 0, 0, 0); \
+            }                                                                        \
+      } else {                                                                       \
         const uint32_t bm = blkmask[S] >> (2 * wc);                                  \
         _Pragma("unroll") for (int h = 0; h < 4; ++h)                                \
           _Pragma("unroll") for (int q = 0; q < 2; ++q)                              \
@@ -821,17 +885,33 @@ __global__ __launch_bounds__(256, 4) void numeric_f64_half_kernel(
     double* Ct = lds;
     for (int p = 0; p < 2; ++p) {
       if ((wave >> 1) == p) {
+        if constexpr (kFine) {
+          // v_mfma_f64_4x4x4_4b result: element (i, j) of block b in lane
+          // 16 i + 4 b + j, block b = rows 4b.. of the 16-row group
+          const int fr = 4 * ((lane >> 2) & 3) + (lane >> 4), fc = lane & 3;
   #pragma unroll
-        for (int h = 0; h < 4; ++h)
+          for (int h = 0; h < 4; ++h)
   #pragma unroll
-          for (int q = 0; q < 2; ++q)
+            for (int q = 0; q < 2; ++q)
   #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const int idx =
-                  (16 * h + 4 * r + (lane >> 4)) * kPitch + 32 * wc + 16 * q + (lane & 15);
-              const double v = acc[2 * h + q][r];
-              Ct[idx] = p == 0 ? v : Ct[idx] + v;
-            }
+              for (int c = 0; c < 4; ++c) {
+                const int idx = (16 * h + fr) * kPitch + 32 * wc + 16 * q + 4 * c + fc;
+                const double v = accf[h][q][c];
+                Ct[idx] = p == 0 ? v : Ct[idx] + v;
+              }
+        } else {
+  #pragma unroll
+          for (int h = 0; h < 4; ++h)
+  #pragma unroll
+            for (int q = 0; q < 2; ++q)
+  #pragma unroll
+              for (int r = 0; r < 4; ++r) {
+                const int idx =
+                    (16 * h + 4 * r + (lane >> 4)) * kPitch + 32 * wc + 16 * q + (lane & 15);
+                const double v = acc[2 * h + q][r];
+                Ct[idx] = p == 0 ? v : Ct[idx] + v;
+              }
+        }
       }
       __syncthreads();
     }
@@ -1178,13 +1258,18 @@ hipError_t launch_numeric_f64(hipStream_t s, int32_t n_tiles, const TileDesc* ti
   else if (variant == kF64Lanes && whole && lanes_panels_fit(a_zero, b_zero))
     form != kFormGrid ? go(numeric_f64_lanes_kernel<true>)
                : go(numeric_f64_lanes_kernel<false>);
-  else if (variant == kF64Half && whole && lanes_panels_fit(a_zero, b_zero))
+  else if (variant == kF64Fine && whole && lanes_panels_fit(a_zero, b_zero))
+    form != kFormGrid ? go(numeric_f64_half_kernel<true, true, true, true>)
+               : go(numeric_f64_half_kernel<true, false, true, true>);
+  else if ((variant == kF64Half || variant == kF64Fine) && whole &&
+           lanes_panels_fit(a_zero, b_zero))
     form != kFormGrid ? go(numeric_f64_half_kernel<true, true, true>)
                : go(numeric_f64_half_kernel<true, false, true>);
-  else if (variant == kF64Half && whole)
+  else if ((variant == kF64Half || variant == kF64Fine) && whole)
     form != kFormGrid ? go(numeric_f64_half_kernel<true, true>)
                : go(numeric_f64_half_kernel<true, false>);
-  else if (variant == kF64Half || variant == kF64Lanes)  // the lanes kernel: whole products
+  else if (variant == kF64Half || variant == kF64Lanes || variant == kF64Fine)
+    // the lanes kernel and the fine form: whole products only
     form != kFormGrid ? go(numeric_f64_half_kernel<false, true>)
                : go(numeric_f64_half_kernel<false, false>);
   else

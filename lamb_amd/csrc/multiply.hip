@@ -114,13 +114,14 @@ static hipError_t record_event(lamb_hip_ctx* ctx, int k) {
 }
 
 // FP64 kernel variant forced by LAMB_HIP_F64_KERNEL (quadrant | half | ksplit |
-// lanes), or -1
+// lanes | fine), or -1
 static int f64_forced() {
   const char* v = getenv("LAMB_HIP_F64_KERNEL");
   if (v && strcmp(v, "quadrant") == 0) return kF64Quadrant;
   if (v && strcmp(v, "ksplit") == 0) return kF64Ksplit;
   if (v && strcmp(v, "half") == 0) return kF64Half;
   if (v && strcmp(v, "lanes") == 0) return kF64Lanes;
+  if (v && strcmp(v, "fine") == 0) return kF64Fine;
   return -1;
 }
 
@@ -320,7 +321,7 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
   // the lanes kernel serves whole products with panels below 4 GB; any other
   // product forced or tuned to it runs the half kernel, and reports it
   const bool lanes_ok = whole && lanes_panels_fit(a_zero, b_zero);
-  if (variant == kF64Lanes && !lanes_ok) variant = kF64Half;
+  if ((variant == kF64Lanes || variant == kF64Fine) && !lanes_ok) variant = kF64Half;
   lamb_hip_ctx::TuneChoice choice;
   bool tune = false;
   const char* env_rb = getenv("LAMB_HIP_TILE_ROWBLOCK");
@@ -385,8 +386,8 @@ int32_t lamb::multiply_masked(lamb_hip_ctx* ctx, const lamb_hip_mat* A_, const l
     // workgroup per tile; the fastest becomes the class's shape
     float best = 0.f;
     int best_v = variant;
-    for (int v : {kF64Half, kF64Quadrant, kF64Ksplit, kF64Lanes}) {
-      if (v == kF64Lanes && !lanes_ok) continue;
+    for (int v : {kF64Half, kF64Quadrant, kF64Ksplit, kF64Lanes, kF64Fine}) {
+      if ((v == kF64Lanes || v == kF64Fine) && !lanes_ok) continue;
       variant = v;
       float ms = 0.f;
       HIPCHK(ctx, timed(choice.form, &ms), "numeric kernel (shape tuning)");
@@ -738,7 +739,8 @@ extern "C" int32_t lamb_hip_tuning_load(lamb_hip_ctx* ctx, const char* path) {
     char tail = 0;
     if (sscanf(line, "shape %lld %lld %lld %lld %lld %d %c", &k[0], &k[1], &k[2], &k[3], &k[4],
                &a, &tail) == 6 &&
-        (a == kF64Quadrant || a == kF64Ksplit || a == kF64Half || a == kF64Lanes)) {
+        (a == kF64Quadrant || a == kF64Ksplit || a == kF64Half || a == kF64Lanes ||
+         a == kF64Fine)) {
       shapes[{k[0], k[1], k[2], k[3], k[4]}] = a;
     } else if (sscanf(line, "order %lld %lld %lld %lld %d %d %c", &k[0], &k[1], &k[2], &k[3], &a,
                       &b, &tail) == 6 &&

@@ -330,7 +330,7 @@ def test_default_kernel_is_deterministic_and_variants_agree(monkeypatch):
     monkeypatch.delenv("LAMB_HIP_TILE_ROWBLOCK")
     fresh = D.Context(0)
     try:
-        for name in ("quadrant", "half", "ksplit", "lanes"):
+        for name in ("quadrant", "half", "ksplit", "lanes", "fine"):
             monkeypatch.setenv("LAMB_HIP_F64_KERNEL", name)
             g, _ = gpu_multiply(fresh, atoms, a, b, eps=1e-7)
             same, err = compare(g, ref)
@@ -339,7 +339,7 @@ def test_default_kernel_is_deterministic_and_variants_agree(monkeypatch):
         fresh.close()
 
 
-@pytest.mark.parametrize("variant", ["quadrant", "half", "ksplit", "lanes"])
+@pytest.mark.parametrize("variant", ["quadrant", "half", "ksplit", "lanes", "fine"])
 def test_forced_variants_whole_and_partial_products(ctx, monkeypatch, variant):
     """Each FP64 variant on a product whose entries are all whole (unmasked
     loads) and on one where the prefilter cuts entries (per-lane masks)."""
@@ -406,7 +406,7 @@ def test_block_norm_order_near_threshold(ctx, size):
 
 
 @pytest.mark.parametrize("sizes", [(64,), (33, 31), (47, 17), (63, 2, 1)])
-@pytest.mark.parametrize("variant", ["quadrant", "half", "ksplit", "lanes", "bf16"])
+@pytest.mark.parametrize("variant", ["quadrant", "half", "ksplit", "lanes", "fine", "bf16"])
 def test_blocks_at_the_band_limit(ctx, monkeypatch, variant, sizes):
     """Atoms of up to 64 rows, the band height: bands of one 64-row atom, of
     two atoms that fill 64 rows exactly, and of many 1- and 2-row atoms (16
