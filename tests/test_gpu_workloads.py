@@ -30,30 +30,41 @@ from lamb_amd.matrix import Distribution, LambMatrix
 pytestmark = pytest.mark.gpu
 
 
-def test_water6144_matches_oracle(ctx):
+def test_water6144_matches_oracle(ctx, monkeypatch):
+    """The default kernel, then the opt-in vector-ALU (lanes) and 4 x 4 x 4
+    (fine) variants, each against the one oracle product."""
     atoms, a, b, eps = S.config2(2)
     print(f"case: 6,144-atom water box, A {a.n_blocks} blocks, eps {eps}", flush=True)
     dist = Distribution(atoms.blk, atoms.pos, atoms.box)
     dd = D.DeviceDistribution(ctx, dist)
     da = D.DeviceMatrix.from_lamb(dd, LambMatrix.from_global(dist, a))
     db = D.DeviceMatrix.from_lamb(dd, LambMatrix.from_global(dist, b))
-    dc = D.DeviceMatrix(dd)
-    st = D.multiply(da, db, dc, eps_filter=eps)
-    got = dc.download().to_global()
-    for h in (dc, db, da, dd):
+    runs = []
+    for variant in (None, "lanes", "fine"):
+        if variant is None:
+            monkeypatch.delenv("LAMB_HIP_F64_KERNEL", raising=False)
+        else:
+            monkeypatch.setenv("LAMB_HIP_F64_KERNEL", variant)
+        dc = D.DeviceMatrix(dd)
+        st = D.multiply(da, db, dc, eps_filter=eps)
+        runs.append((variant or "default", dc.download().to_global(), st))
+        dc.close()
+        print(f"  gpu ({runs[-1][0]}): {st['n_triples_kept']} triples kept, "
+              f"{st['n_c_blocks_kept']} C blocks, numeric {st['ms_numeric']:.2f} ms", flush=True)
+    monkeypatch.delenv("LAMB_HIP_F64_KERNEL", raising=False)
+    for h in (db, da, dd):
         h.close()
-    print(f"  gpu: {st['n_triples_kept']} triples kept, {st['n_c_blocks_kept']} C blocks, "
-          f"numeric {st['ms_numeric']:.2f} ms", flush=True)
     ref, rst = oracle.multiply(a, b, eps=eps)
-    same, err = compare(got, ref)
-    print(f"  oracle: {rst['n_triples_kept']} triples kept, {ref.n_blocks} C blocks; "
-          f"pattern identical {same}, relative error {err:.3e}", flush=True)
-    assert same
-    assert err <= 1e-12
-    assert st["n_triples_kept"] == rst["n_triples_kept"]
-    assert st["n_triples_filtered"] == rst["n_triples_filtered"]
-    assert st["flops"] == rst["flops"]
-    assert st["n_c_blocks_kept"] == ref.n_blocks
+    for name, got, st in runs:
+        same, err = compare(got, ref)
+        print(f"  oracle vs {name}: {rst['n_triples_kept']} triples kept, {ref.n_blocks} C "
+              f"blocks; pattern identical {same}, relative error {err:.3e}", flush=True)
+        assert same, name
+        assert err <= 1e-12, (name, err)
+        assert st["n_triples_kept"] == rst["n_triples_kept"]
+        assert st["n_triples_filtered"] == rst["n_triples_filtered"]
+        assert st["flops"] == rst["flops"]
+        assert st["n_c_blocks_kept"] == ref.n_blocks
 
 
 def _batch_distribution(atoms):
